@@ -1,0 +1,727 @@
+// dps_api.hip -- C ABI of the doubly periodic Stokes solver (see
+// include/dpstokes_hip.h): parameter validation, constant tables, device
+// memory, rocFFT plans and the Mdot launch sequence.
+//
+// Launch sequence of one Mdot (reference FCM.Mdot, FCM.py:334-441):
+//   spread F (+ curl of T)  ->  rocFFT r2c (x-y)  ->  z Chebyshev forward
+//   ->  per-mode FP64 BVP (+ wall-correction coefficients, k = 0 mode)
+//   ->  z Chebyshev inverse (+ correction profiles)  ->  rocFFT c2r
+//   ->  interpolate U (+ derivative kernels for Omega)
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/dpstokes_hip.h"
+#include "dps_internal.h"
+#include "dps_tables.h"
+
+using namespace dps;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                       \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      return fail(DPS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define FFTCHK(x)                                                                       \
+  do {                                                                                  \
+    rocfft_status s_ = (x);                                                             \
+    if (s_ != rocfft_status_success)                                                    \
+      return fail(DPS_ERR_FFT, std::string(#x) + " failed, status " + std::to_string((int)s_)); \
+  } while (0)
+
+int g_rocfft_users = 0;
+
+}  // namespace
+
+struct dps_handle {
+  dps_params P;
+  int N = 0;
+  int geometry = 0, prec = 0;
+  double hx = 0, hy = 0, Lz = 0, H = 0;
+  int Nx = 0, Ny = 0, Nz = 0, Nkx = 0, Nm = 0;
+  KernelDesc km{}, kd{};
+  bool has_dip = false;
+  ZExt zm, zd;
+  std::vector<double> zpts, zwts;
+  ChebTables cheb;
+  ChebTransform ct;
+  int TX = 16, TY = 4, ZC = 32, ntx = 0, nty = 0, ntiles = 0, RX = 1, RY = 1;
+  size_t tsz = 8;   // bytes per grid real
+
+  std::vector<void*> allocs;
+  double* d_pos = nullptr;
+  int* d_err = nullptr;
+  int* h_err = nullptr;
+  int4 *d_sup_m = nullptr, *d_sup_d = nullptr;
+  void *d_rec_m = nullptr, *d_rec_d = nullptr;
+  int rs_m = 0, rs_d = 0;
+  int *d_key = nullptr, *d_rank = nullptr, *d_counts = nullptr, *d_start = nullptr, *d_order = nullptr;
+  void* d_grid = nullptr;
+  void *d_specA = nullptr, *d_specB = nullptr;
+  double *d_zext_m = nullptr, *d_zw_m = nullptr, *d_zext_d = nullptr, *d_zw_d = nullptr;
+  double *d_fA_m = nullptr, *d_fB_m = nullptr, *d_fA_d = nullptr, *d_fB_d = nullptr;
+  int *d_kt_m = nullptr, *d_kt_d = nullptr;
+  void *d_TfE = nullptr, *d_TfO = nullptr, *d_TiE = nullptr, *d_TiO = nullptr;
+  double *d_zpts = nullptr, *d_kxs = nullptr, *d_kys = nullptr;
+  double *d_sd0 = nullptr, *d_sm2 = nullptr, *d_sp2 = nullptr, *d_fm1 = nullptr, *d_fp1 = nullptr;
+  double *d_R1 = nullptr, *d_R2 = nullptr, *d_L1 = nullptr, *d_L2 = nullptr;
+  int *d_is_special = nullptr, *d_special = nullptr;
+  int nspecial = 0;
+  double2* d_u2 = nullptr;
+  double *s_beta = nullptr, *s_ab = nullptr;
+  double2 *s_P = nullptr, *s_CP = nullptr, *s_DP = nullptr, *s_U = nullptr;
+  double2* d_corr = nullptr;
+  double* d_stage = nullptr;   // FP64 reference-layout staging for stage entry points
+
+  rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* fft_work = nullptr;
+  size_t fft_work_size = 0;
+
+  bool positions_set = false;
+  bool prof = false;
+  hipEvent_t ev[9] = {};
+  bool have_timings = false;
+
+  template <class X>
+  int alloc(X** p, size_t count)
+  {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(X));
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(DPS_ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(X)) + " bytes failed");
+    }
+    allocs.push_back((void*)*p);
+    return DPS_OK;
+  }
+  template <class X>
+  int upload(X** p, const std::vector<X>& v)
+  {
+    int rc = alloc(p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) {
+      hipError_t e = hipMemcpy(*p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return fail(DPS_ERR_HIP, "upload failed");
+    }
+    return DPS_OK;
+  }
+  void release()
+  {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (h_err) { (void)hipHostFree(h_err); h_err = nullptr; }
+    if (plan_fwd) rocfft_plan_destroy(plan_fwd);
+    if (plan_inv) rocfft_plan_destroy(plan_inv);
+    if (info) rocfft_execution_info_destroy(info);
+    plan_fwd = plan_inv = nullptr;
+    info = nullptr;
+    for (auto& e : ev)
+      if (e) { (void)hipEventDestroy(e); e = nullptr; }
+  }
+};
+
+namespace {
+
+int setup_kernel(dps_handle* h, int w, double alpha, double beta, bool dipole, KernelDesc* K)
+{
+  if (w < 1) return fail(DPS_ERR_ARG, "kernel width must be >= 1");
+  const double betaw = (beta > 0) ? beta : 1.785 * w;
+  const double a = (alpha > 0) ? alpha : w * h->hx / 2;
+  K->betaw = betaw;
+  K->alpha = a;
+  const double norm = es_norm(a, betaw);
+  K->inv_norm3 = 1.0 / (norm * (norm * norm));
+  K->thresh = dipole ? dipole_thresh(a, betaw) : a;
+  if (!(K->thresh > 0) || !std::isfinite(K->thresh)) return fail(DPS_ERR_ARG, "invalid kernel cutoff");
+  K->wx = (int)std::lround(2 * K->thresh / h->hx);
+  K->wy = (int)std::lround(2 * K->thresh / h->hy);
+  if (K->wx < 1 || K->wy < 1 || K->wx > DPS_MAX_W || K->wy > DPS_MAX_W)
+    return fail(DPS_ERR_ARG, "kernel support of " + std::to_string(K->wx) + "x" +
+                                 std::to_string(K->wy) + " points is out of range [1, 32]");
+  return DPS_OK;
+}
+
+int build(dps_handle* h)
+{
+  const dps_params& P = h->P;
+  h->geometry = P.mode;
+  h->prec = P.precision;
+  h->Nx = P.nx;
+  h->Ny = P.ny;
+  h->Nz = P.nz;
+  h->hx = P.Lx / P.nx;
+  h->hy = P.Ly / P.ny;
+  h->Lz = P.zmax - P.zmin;
+  h->H = h->Lz / 2;
+  h->Nkx = P.nx / 2 + 1;
+  h->Nm = h->Ny * h->Nkx;
+  h->tsz = (h->prec == DPS_PREC_MIXED) ? 4 : 8;
+  int rc = setup_kernel(h, P.w, P.alpha, P.beta, false, &h->km);
+  if (rc) return rc;
+  h->has_dip = P.w_d > 0;
+  if (h->has_dip) {
+    rc = setup_kernel(h, P.w_d, P.alpha_d, P.beta_d, true, &h->kd);
+    if (rc) return rc;
+  }
+  const int wmax = std::max(std::max(h->km.wx, h->km.wy), h->has_dip ? std::max(h->kd.wx, h->kd.wy) : 0);
+  if (h->Nx < 2 * wmax || h->Ny < 2 * wmax)
+    return fail(DPS_ERR_ARG, "nx and ny must be at least twice the kernel support");
+  // z grid: Chebyshev nodes on [0, Lz] (FCM.py:220)
+  clencurt(h->zpts, h->zwts, 0, h->Lz, h->Nz);
+  const bool wall_top = (h->geometry == DPS_MODE_SLIT);
+  const bool wall_bot = (h->geometry == DPS_MODE_BOTTOM || h->geometry == DPS_MODE_SLIT);
+  h->zm = build_zext(h->zpts, h->zwts, h->km.thresh, wall_top, wall_bot);
+  h->km.nzeff = h->zm.nzeff;
+  h->km.ext_up = h->zm.ext_up;
+  h->km.wzmax = h->zm.wzmax;
+  if (h->zm.ext_up != h->zm.ext_down || h->zm.ext_up >= h->Nz)
+    return fail(DPS_ERR_ARG, "kernel support is too wide for the z grid");
+  if (h->has_dip) {
+    h->zd = build_zext(h->zpts, h->zwts, h->kd.thresh, wall_top, wall_bot);
+    h->kd.nzeff = h->zd.nzeff;
+    h->kd.ext_up = h->zd.ext_up;
+    h->kd.wzmax = h->zd.wzmax;
+    if (h->zd.ext_up != h->zd.ext_down || h->zd.ext_up >= h->Nz)
+      return fail(DPS_ERR_ARG, "dipole kernel support is too wide for the z grid");
+  }
+  h->cheb = build_cheb_tables(h->Nz, h->H);
+  h->ct = build_cheb_transform(h->Nz);
+  {
+    ZArgs Z;
+    Z.N = h->Nz;
+    Z.NH = h->ct.NH;
+    if (z_lds_bytes(Z, h->prec == DPS_PREC_MIXED) > 160 * 1024)
+      return fail(DPS_ERR_ARG, "nz is too large for the z-transform kernels");
+  }
+  // tiles for spreading
+  h->TX = std::min(16, h->Nx);
+  h->TY = std::min(4, h->Ny);
+  h->ZC = std::min(32, h->Nz);
+  h->ntx = (h->Nx + h->TX - 1) / h->TX;
+  h->nty = (h->Ny + h->TY - 1) / h->TY;
+  h->ntiles = h->ntx * h->nty;
+  h->RX = (wmax + 1 + h->TX - 1) / h->TX + ((h->Nx % h->TX) ? 1 : 0);
+  h->RY = (wmax + 1 + h->TY - 1) / h->TY + ((h->Ny % h->TY) ? 1 : 0);
+  return DPS_OK;
+}
+
+template <class T>
+std::vector<T> cast_vec(const std::vector<double>& v)
+{
+  std::vector<T> r(v.size());
+  for (size_t i = 0; i < v.size(); ++i) r[i] = (T)v[i];
+  return r;
+}
+
+int allocate(dps_handle* h)
+{
+  int rc;
+  const int N = h->N;
+  const size_t ng = (size_t)3 * h->Nz * h->Ny * h->Nx;
+  const size_t nspec = (size_t)3 * h->Nz * h->Nm;
+  const size_t ts = h->tsz;
+  if ((rc = h->alloc(&h->d_pos, (size_t)3 * N))) return rc;
+  if ((rc = h->alloc(&h->d_err, 1))) return rc;
+  if (hipHostMalloc((void**)&h->h_err, sizeof(int)) != hipSuccess) return fail(DPS_ERR_ALLOC, "pinned alloc failed");
+  if ((rc = h->alloc(&h->d_sup_m, N))) return rc;
+  h->rs_m = h->km.wx + h->km.wy + 2 * h->km.wzmax;
+  if ((rc = h->alloc((unsigned char**)&h->d_rec_m, (size_t)N * h->rs_m * ts))) return rc;
+  if (h->has_dip) {
+    if ((rc = h->alloc(&h->d_sup_d, N))) return rc;
+    h->rs_d = 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
+    if ((rc = h->alloc((unsigned char**)&h->d_rec_d, (size_t)N * h->rs_d * ts))) return rc;
+  }
+  if ((rc = h->alloc(&h->d_key, N))) return rc;
+  if ((rc = h->alloc(&h->d_rank, N))) return rc;
+  if ((rc = h->alloc(&h->d_order, N))) return rc;
+  if ((rc = h->alloc(&h->d_counts, h->ntiles))) return rc;
+  if ((rc = h->alloc(&h->d_start, h->ntiles + 1))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_grid, ng * ts))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_specA, nspec * 2 * ts))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_specB, nspec * 2 * ts))) return rc;
+  // z tables
+  if ((rc = h->upload(&h->d_zext_m, h->zm.z))) return rc;
+  if ((rc = h->upload(&h->d_zw_m, h->zm.w))) return rc;
+  if ((rc = h->upload(&h->d_kt_m, h->zm.kt))) return rc;
+  if ((rc = h->upload(&h->d_fA_m, h->zm.facA))) return rc;
+  if ((rc = h->upload(&h->d_fB_m, h->zm.facB))) return rc;
+  if (h->has_dip) {
+    if ((rc = h->upload(&h->d_zext_d, h->zd.z))) return rc;
+    if ((rc = h->upload(&h->d_zw_d, h->zd.w))) return rc;
+    if ((rc = h->upload(&h->d_kt_d, h->zd.kt))) return rc;
+    if ((rc = h->upload(&h->d_fA_d, h->zd.facA))) return rc;
+    if ((rc = h->upload(&h->d_fB_d, h->zd.facB))) return rc;
+  }
+  // transform matrices in grid precision, inverse scaled by 1/(Nx Ny) (the
+  // c2r transform is unnormalised; Transform.cpp:286-300 divides by Nx Ny Nu)
+  const double inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
+  std::vector<double> tie = h->ct.TiE, tio = h->ct.TiO;
+  for (auto& v : tie) v *= inv_nxy;
+  for (auto& v : tio) v *= inv_nxy;
+  if (h->prec == DPS_PREC_MIXED) {
+    if ((rc = h->upload((float**)&h->d_TfE, cast_vec<float>(h->ct.TfE)))) return rc;
+    if ((rc = h->upload((float**)&h->d_TfO, cast_vec<float>(h->ct.TfO)))) return rc;
+    if ((rc = h->upload((float**)&h->d_TiE, cast_vec<float>(tie)))) return rc;
+    if ((rc = h->upload((float**)&h->d_TiO, cast_vec<float>(tio)))) return rc;
+  } else {
+    if ((rc = h->upload((double**)&h->d_TfE, h->ct.TfE))) return rc;
+    if ((rc = h->upload((double**)&h->d_TfO, h->ct.TfO))) return rc;
+    if ((rc = h->upload((double**)&h->d_TiE, tie))) return rc;
+    if ((rc = h->upload((double**)&h->d_TiO, tio))) return rc;
+  }
+  if ((rc = h->upload(&h->d_zpts, h->zpts))) return rc;
+  std::vector<double> kxs(h->Nkx), kys(h->Ny);
+  for (int i = 0; i < h->Nkx; ++i) {
+    const int n = (i <= h->Nx / 2) ? i : i - h->Nx;
+    kxs[i] = 2 * M_PI * (double)n / h->P.Lx;
+  }
+  for (int i = 0; i < h->Ny; ++i) {
+    const int n = (i <= h->Ny / 2) ? i : i - h->Ny;
+    kys[i] = 2 * M_PI * (double)n / h->P.Ly;
+  }
+  if ((rc = h->upload(&h->d_kxs, kxs))) return rc;
+  if ((rc = h->upload(&h->d_kys, kys))) return rc;
+  // Chebyshev bands and BC rows
+  if ((rc = h->upload(&h->d_sd0, h->cheb.sd0))) return rc;
+  if ((rc = h->upload(&h->d_sm2, h->cheb.sm2))) return rc;
+  if ((rc = h->upload(&h->d_sp2, h->cheb.sp2))) return rc;
+  if ((rc = h->upload(&h->d_fm1, h->cheb.fm1))) return rc;
+  if ((rc = h->upload(&h->d_fp1, h->cheb.fp1))) return rc;
+  if ((rc = h->upload(&h->d_R1, h->cheb.R1))) return rc;
+  if ((rc = h->upload(&h->d_R2, h->cheb.R2))) return rc;
+  if ((rc = h->upload(&h->d_L1, h->cheb.L1))) return rc;
+  if ((rc = h->upload(&h->d_L2, h->cheb.L2))) return rc;
+  // modes that touch a Nyquist index
+  std::vector<int> is_sp(h->Nm, 0), sp;
+  for (int iy = 0; iy < h->Ny; ++iy)
+    for (int ix = 0; ix < h->Nkx; ++ix) {
+      const bool s = ((h->Nx % 2 == 0) && ix == h->Nx / 2) || ((h->Ny % 2 == 0) && iy == h->Ny / 2);
+      if (s && !(ix == 0 && iy == 0)) {
+        is_sp[iy * h->Nkx + ix] = 1;
+        sp.push_back(iy * h->Nkx + ix);
+      }
+    }
+  h->nspecial = (int)sp.size();
+  if ((rc = h->upload(&h->d_is_special, is_sp))) return rc;
+  if ((rc = h->upload(&h->d_special, sp))) return rc;
+  if ((rc = h->alloc(&h->d_u2, (size_t)std::max(1, h->nspecial) * 3 * h->Nz))) return rc;
+  // BVP scratch and correction coefficients
+  const size_t nm = (size_t)h->Nm;
+  if ((rc = h->alloc(&h->s_beta, (size_t)(h->Nz + 1) * nm))) return rc;
+  if ((rc = h->alloc(&h->s_ab, (size_t)h->Nz * nm))) return rc;
+  if ((rc = h->alloc(&h->s_P, (size_t)h->Nz * nm))) return rc;
+  if ((rc = h->alloc(&h->s_CP, (size_t)h->Nz * nm))) return rc;
+  if ((rc = h->alloc(&h->s_DP, (size_t)h->Nz * nm))) return rc;
+  if ((rc = h->alloc(&h->s_U, (size_t)h->Nz * nm))) return rc;
+  if ((rc = h->alloc(&h->d_corr, nm * 12))) return rc;
+  // rocFFT plans: 2-D over (x, y), batched over (component, z)
+  const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
+  size_t lengths[2] = {(size_t)h->Nx, (size_t)h->Ny};
+  FFTCHK(rocfft_plan_create(&h->plan_fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                            fp, 2, lengths, (size_t)3 * h->Nz, nullptr));
+  FFTCHK(rocfft_plan_create(&h->plan_inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                            fp, 2, lengths, (size_t)3 * h->Nz, nullptr));
+  size_t w1 = 0, w2 = 0;
+  FFTCHK(rocfft_plan_get_work_buffer_size(h->plan_fwd, &w1));
+  FFTCHK(rocfft_plan_get_work_buffer_size(h->plan_inv, &w2));
+  h->fft_work_size = std::max(w1, w2);
+  if (h->fft_work_size)
+    if ((rc = h->alloc((unsigned char**)&h->fft_work, h->fft_work_size))) return rc;
+  FFTCHK(rocfft_execution_info_create(&h->info));
+  if (h->fft_work_size) FFTCHK(rocfft_execution_info_set_work_buffer(h->info, h->fft_work, h->fft_work_size));
+  return DPS_OK;
+}
+
+bool mixed(const dps_handle* h) { return h->prec == DPS_PREC_MIXED; }
+
+int run_fft(dps_handle* h, bool forward, void* in, void* out, hipStream_t s)
+{
+  FFTCHK(rocfft_execution_info_set_stream(h->info, s));
+  void* ib[1] = {in};
+  void* ob[1] = {out};
+  FFTCHK(rocfft_execute(forward ? h->plan_fwd : h->plan_inv, ib, ob, h->info));
+  return DPS_OK;
+}
+
+StencilArgs stencil_args(dps_handle* h)
+{
+  StencilArgs A;
+  A.N = h->N;
+  A.xmin = 0;
+  A.ymin = 0;
+  A.zmin = h->P.zmin;
+  A.Lx = h->P.Lx;
+  A.Ly = h->P.Ly;
+  A.Lz = h->Lz;
+  A.hx = h->hx;
+  A.hy = h->hy;
+  A.Nx = h->Nx;
+  A.Ny = h->Ny;
+  A.has_dip = h->has_dip;
+  A.km = h->km;
+  A.kd = h->kd;
+  A.zext_m = h->d_zext_m;
+  A.zw_m = h->d_zw_m;
+  A.zext_d = h->d_zext_d;
+  A.zw_d = h->d_zw_d;
+  A.pos = h->d_pos;
+  A.err = h->d_err;
+  A.sup_m = h->d_sup_m;
+  A.sup_d = h->d_sup_d;
+  A.rec_m = h->d_rec_m;
+  A.rec_d = h->d_rec_d;
+  A.rs_m = h->rs_m;
+  A.rs_d = h->rs_d;
+  A.key = h->d_key;
+  A.TX = h->TX;
+  A.TY = h->TY;
+  A.ntx = h->ntx;
+  return A;
+}
+
+SpreadArgs spread_args(dps_handle* h)
+{
+  SpreadArgs A;
+  A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
+  A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty; A.RX = h->RX; A.RY = h->RY;
+  A.start = h->d_start; A.order = h->d_order;
+  A.km = h->km; A.kd = h->kd;
+  A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
+  A.rec_m = h->d_rec_m; A.rec_d = h->d_rec_d; A.rs_m = h->rs_m; A.rs_d = h->rs_d;
+  A.kt_m = h->d_kt_m; A.kt_d = h->d_kt_d;
+  A.fA_m = h->d_fA_m; A.fB_m = h->d_fB_m; A.fA_d = h->d_fA_d; A.fB_d = h->d_fB_d;
+  A.grid = h->d_grid;
+  return A;
+}
+
+InterpArgs interp_args(dps_handle* h)
+{
+  InterpArgs A;
+  A.N = h->N; A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
+  A.order = h->d_order;
+  A.km = h->km; A.kd = h->kd;
+  A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
+  A.rec_m = h->d_rec_m; A.rec_d = h->d_rec_d; A.rs_m = h->rs_m; A.rs_d = h->rs_d;
+  A.kt_m = h->d_kt_m; A.kt_d = h->d_kt_d;
+  A.fA_m = h->d_fA_m; A.fA_d = h->d_fA_d; A.fB_d = h->d_fB_d;
+  A.grid = h->d_grid;
+  return A;
+}
+
+ZArgs z_args(dps_handle* h)
+{
+  ZArgs A;
+  A.N = h->Nz; A.NH = h->ct.NH; A.NE = h->ct.NE; A.NO = h->ct.NO;
+  A.Nm = h->Nm; A.Nkx = h->Nkx;
+  A.TfE = h->d_TfE; A.TfO = h->d_TfO; A.TiE = h->d_TiE; A.TiO = h->d_TiO;
+  A.nbasis = (h->geometry == DPS_MODE_BOTTOM) ? 2 : (h->geometry == DPS_MODE_SLIT ? 4 : 0);
+  A.corr = h->d_corr;
+  A.kxs = h->d_kxs; A.kys = h->d_kys; A.zpts = h->d_zpts;
+  A.Hs = h->Lz;
+  A.inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
+  return A;
+}
+
+BvpArgs bvp_args(dps_handle* h, bool special)
+{
+  BvpArgs A;
+  BvpConsts& B = A.B;
+  B.Nz = h->Nz; B.geometry = h->geometry; B.H = h->H; B.Hs = h->Lz; B.eta = h->P.viscosity;
+  B.sd0 = h->d_sd0; B.sm2 = h->d_sm2; B.sp2 = h->d_sp2;
+  B.S0N = h->cheb.S0N; B.S1N1 = h->cheb.S1N1;
+  B.fm1 = h->d_fm1; B.fp1 = h->d_fp1; B.F0N1 = h->cheb.F0N1;
+  B.R1 = h->d_R1; B.R2 = h->d_R2; B.L1 = h->d_L1; B.L2 = h->d_L2;
+  A.Nx = h->Nx; A.Ny = h->Ny; A.Nkx = h->Nkx; A.Nm = h->Nm;
+  A.Lx = h->P.Lx; A.Ly = h->P.Ly;
+  A.is_special = h->d_is_special;
+  A.special = special ? h->d_special : nullptr;
+  A.nspecial = h->nspecial;
+  A.u2 = h->d_u2;
+  A.s_beta = h->s_beta; A.s_ab = h->s_ab;
+  A.s_P = h->s_P; A.s_CP = h->s_CP; A.s_DP = h->s_DP; A.s_U = h->s_U;
+  A.corr = h->d_corr;
+  return A;
+}
+
+void mark(dps_handle* h, int i, hipStream_t s)
+{
+  if (h->prof) (void)hipEventRecord(h->ev[i], s);
+}
+
+int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_t s)
+{
+  const SpreadArgs A = spread_args(h);
+  const void* Tq = (T && h->has_dip) ? T : nullptr;
+  if (mixed(h)) {
+    if (dtype == DPS_F32) launch_spread<float, float>(A, F, Tq, s);
+    else launch_spread<float, double>(A, F, Tq, s);
+  } else {
+    if (dtype == DPS_F32) launch_spread<double, float>(A, F, Tq, s);
+    else launch_spread<double, double>(A, F, Tq, s);
+  }
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
+int do_solve(dps_handle* h, hipStream_t s, int ev0)
+{
+  int rc;
+  if ((rc = run_fft(h, true, h->d_grid, h->d_specA, s))) return rc;
+  mark(h, ev0 + 1, s);
+  const ZArgs Z = z_args(h);
+  if (mixed(h)) launch_zfwd<float>(Z, h->d_specA, h->d_specB, s);
+  else launch_zfwd<double>(Z, h->d_specA, h->d_specB, s);
+  HIPCHK(hipGetLastError());
+  mark(h, ev0 + 2, s);
+  for (int sp = 0; sp < 2; ++sp) {
+    const BvpArgs B = bvp_args(h, sp == 1);
+    if (mixed(h)) launch_bvp<float>(B, h->d_specB, h->d_specA, s);
+    else launch_bvp<double>(B, h->d_specB, h->d_specA, s);
+    HIPCHK(hipGetLastError());
+  }
+  mark(h, ev0 + 3, s);
+  if (mixed(h)) launch_zinv<float>(Z, h->d_specA, h->d_specB, s);
+  else launch_zinv<double>(Z, h->d_specA, h->d_specB, s);
+  HIPCHK(hipGetLastError());
+  mark(h, ev0 + 4, s);
+  if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+  mark(h, ev0 + 5, s);
+  return DPS_OK;
+}
+
+int do_interp(dps_handle* h, void* vel, void* ang, int dtype, hipStream_t s)
+{
+  const InterpArgs A = interp_args(h);
+  void* a = (ang && h->has_dip) ? ang : nullptr;
+  if (mixed(h)) {
+    if (dtype == DPS_F32) launch_interp<float, float>(A, vel, a, s);
+    else launch_interp<float, double>(A, vel, a, s);
+  } else {
+    if (dtype == DPS_F32) launch_interp<double, float>(A, vel, a, s);
+    else launch_interp<double, double>(A, vel, a, s);
+  }
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
+int check_handle(const dps_handle* h, bool need_pos)
+{
+  if (!h) return fail(DPS_ERR_STATE, "DPStokes is not initialized. Call Initialize first");
+  if (need_pos && !h->positions_set) return fail(DPS_ERR_STATE, "positions have not been set");
+  return DPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dps_create(const dps_params* params, int number_particles, dps_handle** out)
+{
+  if (!out) return fail(DPS_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (!params) return fail(DPS_ERR_ARG, "null parameters");
+  const dps_params& P = *params;
+  if (P.mode != DPS_MODE_NOWALL && P.mode != DPS_MODE_BOTTOM && P.mode != DPS_MODE_SLIT)
+    return fail(DPS_ERR_ARG, "mode must be nowall, bottom or slit");
+  if (P.precision != DPS_PREC_DOUBLE && P.precision != DPS_PREC_MIXED)
+    return fail(DPS_ERR_ARG, "precision must be DPS_PREC_DOUBLE or DPS_PREC_MIXED");
+  if (number_particles < 0) return fail(DPS_ERR_ARG, "number of particles must be >= 0");
+  if (!(P.Lx > 0) || !(P.Ly > 0) || !(P.zmax > P.zmin) || !(P.viscosity > 0))
+    return fail(DPS_ERR_ARG, "box lengths, z extent and viscosity must be positive");
+  if (P.nx < 4 || P.ny < 4 || P.nz < 5) return fail(DPS_ERR_ARG, "need nx, ny >= 4 and nz >= 5");
+  if ((long)P.nx * P.ny * P.nz > (1L << 31) / 3) return fail(DPS_ERR_ARG, "grid too large");
+  dps_handle* h = new dps_handle();
+  h->P = P;
+  h->N = number_particles;
+  int rc = build(h);
+  bool fft_user = false;
+  if (!rc) {
+    if (g_rocfft_users++ == 0) rocfft_setup();
+    fft_user = true;
+    rc = allocate(h);
+  }
+  if (rc) {
+    const std::string msg = g_err;
+    h->release();
+    delete h;
+    if (fft_user && --g_rocfft_users == 0) rocfft_cleanup();
+    g_err = msg;
+    return rc;
+  }
+  *out = h;
+  return DPS_OK;
+}
+
+int dps_destroy(dps_handle* h)
+{
+  if (!h) return DPS_OK;
+  h->release();
+  delete h;
+  if (g_rocfft_users > 0 && --g_rocfft_users == 0) rocfft_cleanup();
+  return DPS_OK;
+}
+
+int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* stream)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (!positions && h->N > 0) return fail(DPS_ERR_ARG, "null positions");
+  hipStream_t s = (hipStream_t)stream;
+  launch_reset_err(h->d_err, s);
+  const StencilArgs A = stencil_args(h);
+  if (mixed(h)) {
+    if (dtype == DPS_F32) launch_stencil<float, float>(A, positions, s);
+    else launch_stencil<double, float>(A, positions, s);
+  } else {
+    if (dtype == DPS_F32) launch_stencil<float, double>(A, positions, s);
+    else launch_stencil<double, double>(A, positions, s);
+  }
+  HIPCHK(hipGetLastError());
+  launch_sort(h->d_key, h->d_counts, h->d_rank, h->d_start, h->d_order, h->N, h->ntiles, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (*h->h_err) {
+    h->positions_set = false;
+    return fail(DPS_ERR_RANGE, "Position is outside of z extent");
+  }
+  h->positions_set = true;
+  return DPS_OK;
+}
+
+int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* velocities, void* angular,
+             int dtype, void* stream)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (h->N > 0 && (!forces || !velocities)) return fail(DPS_ERR_ARG, "null forces or velocities");
+  if (torques && !h->has_dip) return fail(DPS_ERR_ARG, "torques given but no dipole kernel (w_d = 0)");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->prof) {
+    for (auto& e : h->ev)
+      if (!e) HIPCHK(hipEventCreate(&e));
+  }
+  mark(h, 0, s);
+  if ((rc = do_spread(h, forces, torques, dtype, s))) return rc;
+  mark(h, 1, s);
+  if ((rc = do_solve(h, s, 1))) return rc;
+  if ((rc = do_interp(h, velocities, torques ? angular : nullptr, dtype, s))) return rc;
+  mark(h, 7, s);
+  h->have_timings = h->prof;
+  return DPS_OK;
+}
+
+int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int dtype, double* grid_out,
+                     void* stream)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = do_spread(h, forces, torques, dtype, s))) return rc;
+  if (mixed(h)) launch_grid_to_ref<float>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
+  else launch_grid_to_ref<double>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
+int dps_stage_solve(dps_handle* h, const double* grid_in, double* grid_out, void* stream)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (mixed(h)) launch_ref_to_grid<float>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
+  else launch_ref_to_grid<double>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
+  HIPCHK(hipGetLastError());
+  const bool p = h->prof;
+  h->prof = false;
+  rc = do_solve(h, s, 1);
+  h->prof = p;
+  if (rc) return rc;
+  if (mixed(h)) launch_grid_to_ref<float>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
+  else launch_grid_to_ref<double>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
+int dps_stage_interp(dps_handle* h, const double* grid_in, void* velocities, void* angular, int dtype,
+                     void* stream)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (mixed(h)) launch_ref_to_grid<float>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
+  else launch_ref_to_grid<double>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
+  HIPCHK(hipGetLastError());
+  return do_interp(h, velocities, angular, dtype, s);
+}
+
+int dps_get_info(const dps_handle* h, int* out)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (!out) return fail(DPS_ERR_ARG, "null output");
+  out[0] = h->Nx;
+  out[1] = h->Ny;
+  out[2] = h->Nz;
+  out[3] = h->N;
+  out[4] = h->km.nzeff;
+  out[5] = h->has_dip ? h->kd.nzeff : 0;
+  out[6] = h->km.wzmax;
+  out[7] = h->has_dip ? h->kd.wzmax : 0;
+  return DPS_OK;
+}
+
+int dps_set_profiling(dps_handle* h, int enable)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  h->prof = enable != 0;
+  h->have_timings = false;
+  return DPS_OK;
+}
+
+int dps_get_timings(dps_handle* h, double* out)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (!h->have_timings) return fail(DPS_ERR_STATE, "no profiled Mdot available");
+  HIPCHK(hipEventSynchronize(h->ev[7]));
+  // events: 0 start, 1 spread done, 2 fft fwd, 3 z fwd, 4 bvp, 5 z inv, 6 fft inv, 7 interp
+  out[0] = 0.0;   // locate/sort happens in dps_set_positions
+  for (int i = 0; i < 7; ++i) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+    out[i + 1] = 1000.0 * ms;
+  }
+  return DPS_OK;
+}
+
+const char* dps_get_precision(const dps_handle* h)
+{
+  if (h && h->prec == DPS_PREC_MIXED) return "single";
+  return "double";
+}
+
+const char* dps_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,508 @@
+// dps_core.h -- arithmetic shared by the gfx950 kernels and the host-side
+// checks: ES kernels, per-particle stencils, and the per-wavenumber FP64
+// boundary-value solve of the doubly periodic Stokes problem.
+//
+// Everything here is plain C++ marked __host__ __device__ under hipcc, so the
+// exact code that runs in the kernels can also be executed on the CPU.
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define DPS_HD __host__ __device__ inline
+#else
+#define DPS_HD inline
+#endif
+
+namespace dps {
+
+// ---------------------------------------------------------------------------
+// minimal FP64 complex
+// ---------------------------------------------------------------------------
+struct cplx {
+  double re, im;
+};
+DPS_HD cplx mk(double r, double i) { cplx c; c.re = r; c.im = i; return c; }
+DPS_HD cplx operator+(cplx a, cplx b) { return mk(a.re + b.re, a.im + b.im); }
+DPS_HD cplx operator-(cplx a, cplx b) { return mk(a.re - b.re, a.im - b.im); }
+DPS_HD cplx operator-(cplx a) { return mk(-a.re, -a.im); }
+DPS_HD cplx operator*(cplx a, cplx b) { return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+DPS_HD cplx operator*(double s, cplx a) { return mk(s * a.re, s * a.im); }
+DPS_HD cplx operator*(cplx a, double s) { return mk(a.re * s, a.im * s); }
+DPS_HD cplx operator/(cplx a, double s) { return mk(a.re / s, a.im / s); }
+DPS_HD cplx conjg(cplx a) { return mk(a.re, -a.im); }
+DPS_HD cplx times_i(cplx a) { return mk(-a.im, a.re); }   // i * a
+
+// geometry codes (same values as the C ABI)
+enum { MODE_NOWALL = 1, MODE_BOTTOM = 2, MODE_SLIT = 3 };
+
+// ---------------------------------------------------------------------------
+// ES kernel (reference include/ESKernels.h:8-54)
+//   phi(x)  = exp(bw (sqrt(1 - x^2/a^2) - 1))
+//   phi'(x) = -bw x phi(x) / (a^2 sqrt(1 - x^2/a^2))
+// The 3-D kernel is the product of three 1-D factors.
+// ---------------------------------------------------------------------------
+DPS_HD double es_phi(double x, double bw, double a2)
+{
+  return exp(bw * (sqrt(1 - x * x / a2) - 1));
+}
+DPS_HD double es_dfac(double x, double bw, double a2)
+{
+  return -bw * x / (a2 * sqrt(1 - x * x / a2));
+}
+
+// Kernel description shared by host and device.
+struct KernelDesc {
+  double betaw;      // beta * w
+  double alpha;      // support half width
+  double thresh;     // cutoff (alpha for the monopole, < alpha for the dipole)
+  double inv_norm3;  // 1 / norm^3
+  int wx, wy;        // support points in x and y: round(2 thresh / hx), round(2 thresh / hy)
+  int nzeff;         // points of the z-extended grid
+  int ext_up;        // mirrored ghost nodes above the top
+  int wzmax;         // bound on the z support size
+};
+
+// first index e with zext[e] <= v for descending zext (std::lower_bound with
+// std::greater, as in the reference's ParticleList.cpp:456-459)
+DPS_HD int lower_bound_desc(const double* zext, int n, double v)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (zext[mid] > v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Per-particle stencil: support origin and 1-D weights.
+// Follows ParticleList::locateOnGridNonUnifZ (ParticleList.cpp:453-524):
+// offsets beyond +-thresh are clamped to +thresh (not zeroed), the z support is
+// every extended-grid node within [z - thresh, z + thresh].
+//   x, y, z : position in [0,Lx] x [0,Ly] x [0,Lz]
+//   bx, by  : unwrapped grid index of the first support point
+//   e0, nz  : first extended-grid z index and number of z points
+// Weights written when the pointer is non-null (stride 1):
+//   wx  = phi(x offsets) / norm^3        dx = wx * dfac (derivative)
+//   wy  = phi(y offsets)                 dy
+//   wz  = phi(z offsets)                 dz
+struct Stencil {
+  int bx, by, e0, nz;
+};
+
+DPS_HD Stencil stencil_support(double x, double y, double z, double hx, double hy,
+                               const KernelDesc& k, const double* zext)
+{
+  Stencil s;
+  const int wx = k.wx, wy = k.wy;
+  const int evenx = -1 * (wx % 2) + 1, eveny = -1 * (wy % 2) + 1;
+  int xc = (int)(x / hx), yc = (int)(y / hy);
+  xc += ((wx % 2) && (x / hx - xc > 1.0 / 2.0) ? 1 : 0);
+  yc += ((wy % 2) && (y / hy - yc > 1.0 / 2.0) ? 1 : 0);
+  s.bx = xc - wx / 2 + evenx;
+  s.by = yc - wy / 2 + eveny;
+  const int high = lower_bound_desc(zext, k.nzeff, z - k.thresh);
+  const int low = lower_bound_desc(zext, k.nzeff, z + k.thresh);
+  int ir = high;
+  if (ir == k.nzeff) ir -= 1;
+  else if (z - k.thresh > zext[ir]) ir -= 1;
+  s.e0 = low;
+  s.nz = ir - low + 1;
+  if (s.nz < 0) s.nz = 0;
+  return s;
+}
+
+DPS_HD double clamp_off(double v, double thresh)
+{
+  return (v > thresh || v < -thresh) ? thresh : v;
+}
+
+// 1-D weights along x (with 1/norm^3 folded in) or y (scale = 1); w points.
+DPS_HD void stencil_weights_xy(double pos, double h, int b, int w, const KernelDesc& k,
+                               double scale, double* wout, double* dout)
+{
+  const double a2 = k.alpha * k.alpha;
+  for (int i = 0; i < w; ++i) {
+    const double v = clamp_off((double)(b + i) * h - pos, k.thresh);
+    const double p = es_phi(v, k.betaw, a2) * scale;
+    if (wout) wout[i] = p;
+    if (dout) dout[i] = p * es_dfac(v, k.betaw, a2);
+  }
+}
+
+// 1-D weights along z on the extended grid; wint (optional) also multiplies
+// by the interpolation quadrature weight hx*hy*zw_ext[e] (ParticleList.cpp:521).
+DPS_HD void stencil_weights_z(double z, const Stencil& s, const KernelDesc& k, const double* zext,
+                              const double* zwext, double hxhy, double* wout, double* dout,
+                              double* wint, double* dint)
+{
+  const double a2 = k.alpha * k.alpha;
+  for (int q = 0; q < s.nz; ++q) {
+    const int e = s.e0 + q;
+    const double v = clamp_off(zext[e] - z, k.thresh);
+    const double p = es_phi(v, k.betaw, a2);
+    const double d = p * es_dfac(v, k.betaw, a2);
+    const double qw = hxhy * zwext[e];
+    if (wout) wout[q] = p;
+    if (dout) dout[q] = d;
+    if (wint) wint[q] = p * qw;
+    if (dint) dint[q] = d * qw;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-wavenumber boundary-value solve (FP64).
+//
+// Restates, for one Fourier mode k != 0, the reference chain
+//   chebCoeffDiff -> assemble_p_rhs -> schurSolve_p -> assemble_u_rhs ->
+//   assemble_u_bc_rhs -> schurSolve_u      (DPTools.cpp:81-345)
+// with the per-k operators of Solvers.py:258-347 / LinearSolvers.cpp:11-60
+// rebuilt from |k| on the fly instead of stored per mode, followed by the
+// wall correction of DPTools.cpp:590-877 expressed as coefficients of the
+// analytic profiles e^{-Kz}, e^{K(z-Hs)}, z e^{-Kz}, z e^{K(z-Hs)}.  The
+// profiles themselves are added in value space by the inverse z transform,
+// which is the same linear map as adding their Chebyshev transforms here.
+// ---------------------------------------------------------------------------
+struct BvpConsts {
+  int Nz;
+  int geometry;
+  double H;      // Lz / 2
+  double Hs;     // slit height (Lz)
+  double eta;
+  // second integral matrix band (H^2 scaled): S[j][j], S[j][j-2], S[j][j+2]
+  const double* sd0;
+  const double* sm2;
+  const double* sp2;
+  double S0N, S1N1;   // S[0][Nz], S[1][Nz+1]
+  // first integral matrix band (H scaled): F[j][j-1], F[j][j+1]
+  const double* fm1;
+  const double* fp1;
+  double F0N1;        // F[0][Nz+1]
+  // no-wall BC rows, length Nz+2 (Solvers.py:603-643)
+  const double *R1, *R2, *L1, *L2;
+};
+
+// Mode-local arrays are accessed through this view: element i of array `a`
+// lives at base[a][i * stride] (stride = number of modes, so that
+// neighbouring modes are contiguous).  On the host the same view runs with
+// stride 1 over private buffers.
+struct ScratchView {
+  double* beta;   // Nz+1
+  double* ab;     // Nz  (A^{-1}B, column 0 on even rows, column 1 on odd rows)
+  cplx* P;        // Nz  pressure rhs / solution
+  cplx* CP;       // Nz  pressure coefficients
+  cplx* DP;       // Nz  z-derivative of pressure coefficients
+  cplx* U;        // Nz  velocity work vector
+  long stride;
+};
+
+// Input accessor: coefficient (c, k) of the forcing for this mode.
+template <class In>
+DPS_HD cplx fin(const In& in, int c, int k) { return in(c, k); }
+
+// Output: Chebyshev coefficients of velocity (no wall correction) and the
+// correction coefficients corr[c][b] (b over the profile basis).
+struct ModeResult {
+  cplx corr[3][4];
+  int nbasis;
+};
+
+template <class In, class Out>
+DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int nyq_y,
+                     const In& fin_, Out& fout, const ScratchView& sc, ModeResult& res)
+{
+  const int Nz = B.Nz;
+  const long st = sc.stride;
+  const double H = B.H;
+  const double Ksq = kx * kx + ky * ky;
+  const double K = sqrt(Ksq);
+  const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
+  const cplx Dy = nyq_y ? mk(0, 0) : mk(0, ky);
+#define D0(i) (1.0 - B.sd0[(i)] * Ksq)
+#define DP2(i) (0.0 - B.sp2[(i)] * Ksq)
+#define DM2(i) (0.0 - B.sm2[(i) + 2] * Ksq)
+  // KBPENTA pivots (LinearSolvers.h:52-74)
+  double* beta = sc.beta;
+  beta[0] = 0;
+  beta[1 * st] = D0(0);
+  beta[2 * st] = D0(1);
+  {
+    double bm2 = D0(0), bm1 = D0(1);   // beta[i-2], beta[i-1]
+    for (int i = 3; i <= Nz; ++i) {
+      const double b = D0(i - 1) - DM2(i - 3) * DP2(i - 3) / bm2;
+      beta[(long)i * st] = b;
+      bm2 = bm1;
+      bm1 = b;
+    }
+  }
+  // A^{-1}B (LinearSolvers.cpp:42): B = -Ksq * SIMat[:, Nz:Nz+2] has one
+  // non-zero per column (rows 0 and 1), so the two solutions live on even and
+  // odd rows and are stored merged.
+  double* ab = sc.ab;
+  {
+    double x0 = B.S0N * (-Ksq), x1 = B.S1N1 * (-Ksq);
+    ab[0] = x0;
+    ab[st] = x1;
+    double xm2 = x0, xm1 = x1;
+    for (int i = 2; i < Nz; ++i) {
+      const double v = 0.0 - DM2(i - 2) * xm2 / beta[(long)(i - 1) * st];
+      ab[(long)i * st] = v;
+      xm2 = xm1;
+      xm1 = v;
+    }
+    ab[(long)(Nz - 1) * st] /= beta[(long)Nz * st];
+    ab[(long)(Nz - 2) * st] /= beta[(long)(Nz - 1) * st];
+    double xp2 = ab[(long)(Nz - 1) * st], xp1 = ab[(long)(Nz - 2) * st];
+    for (int i = Nz - 3; i >= 0; --i) {
+      const double v = (ab[(long)i * st] - DP2(i) * xp2) / beta[(long)(i + 1) * st];
+      ab[(long)i * st] = v;
+      xp2 = xp1;
+      xp1 = v;
+    }
+  }
+  // Robin BC rows H^2 K BCR2 + H BCR1 and H^2 K BCL2 + H BCL1 (Solvers.py:309-313)
+  const double H2K = (H * H) * K;
+#define C0(j) (B.R2[(j)] * H2K + (B.R2[(j)] + (H * B.R1[(j)] - B.R2[(j)])))
+#define C1(j) (B.L2[(j)] * H2K + (-B.L2[(j)] + (H * B.L1[(j)] + B.L2[(j)])))
+  double G00 = 0, G01 = 0, G10 = 0, G11 = 0;
+  for (int j = 0; j < Nz; ++j) {
+    const double a = ab[(long)j * st];
+    if (j & 1) { G01 += C0(j) * a; G11 += C1(j) * a; }
+    else { G00 += C0(j) * a; G10 += C1(j) * a; }
+  }
+  G00 -= C0(Nz); G10 -= C1(Nz); G01 -= C0(Nz + 1); G11 -= C1(Nz + 1);
+  const double det = 1.0 / (G00 * G11 - G01 * G10);
+  const double Gi00 = G11 * det, Gi10 = -G10 * det, Gi01 = -G01 * det, Gi11 = G00 * det;
+
+  // ---- pressure: chebCoeffDiff + assemble_p_rhs (DPTools.cpp:134-195) ----
+  cplx* P = sc.P;
+  {
+    cplx dfp1 = mk(0, 0), dfp2 = mk(0, 0);   // df[j+1], df[j+2]
+    for (int j = Nz - 1; j >= 0; --j) {
+      cplx df;
+      if (j == Nz - 1) df = mk(0, 0);
+      else if (j == Nz - 2) df = ((double)((Nz - 1) * 2) / H) * fin(fin_, 2, Nz - 1);
+      else df = dfp2 + ((double)((j + 1) * 2) / H) * fin(fin_, 2, j + 1);
+      dfp2 = dfp1;
+      dfp1 = df;
+      if (j == 0) df = df / 2;
+      P[(long)j * st] = Dx * fin(fin_, 0, j) + Dy * fin(fin_, 1, j) + df;
+    }
+  }
+  // KBPENTA forward / backward sweeps on P (LinearSolvers.h:106-141)
+  for (int i = 2; i < Nz; ++i)
+    P[(long)i * st] = P[(long)i * st] - (DM2(i - 2) * P[(long)(i - 2) * st]) / beta[(long)(i - 1) * st];
+  P[(long)(Nz - 1) * st] = P[(long)(Nz - 1) * st] / beta[(long)Nz * st];
+  P[(long)(Nz - 2) * st] = P[(long)(Nz - 2) * st] / beta[(long)(Nz - 1) * st];
+  for (int i = Nz - 3; i >= 0; --i)
+    P[(long)i * st] = (P[(long)i * st] - DP2(i) * P[(long)(i + 2) * st]) / beta[(long)(i + 1) * st];
+  cplx y0 = mk(0, 0), y1 = mk(0, 0);
+  for (int j = 0; j < Nz; ++j) {
+    const cplx x = P[(long)j * st];
+    y0 = y0 + x * C0(j);
+    y1 = y1 + x * C1(j);
+  }
+  const cplx Y0 = Gi00 * y0 + Gi01 * y1;
+  const cplx Y1 = Gi10 * y0 + Gi11 * y1;
+  // x = xp - A^{-1}B Y ; Cp = SIMat x ; Dp = FIMat x  (DPTools.cpp:266-284)
+  cplx* CP = sc.CP;
+  cplx* DPv = sc.DP;
+  cplx psum = mk(0, 0), palt = mk(0, 0);
+  {
+#define XP(j) ((j) < Nz ? (P[(long)(j) * st] - ((((j) & 1) ? ab[(long)(j) * st] * Y1 : ab[(long)(j) * st] * Y0))) \
+                        : ((j) == Nz ? Y0 : Y1))
+    for (int m = 0; m < Nz; ++m) {
+      cplx cp = B.sd0[m] * XP(m);
+      if (m >= 2) cp = B.sm2[m] * XP(m - 2) + cp;
+      cp = cp + B.sp2[m] * XP(m + 2);
+      if (m == 0) cp = cp + B.S0N * Y0;
+      if (m == 1) cp = cp + B.S1N1 * Y1;
+      cplx dp = B.fp1[m] * XP(m + 1);
+      if (m >= 1) dp = B.fm1[m] * XP(m - 1) + dp;
+      if (m == 0) dp = dp + B.F0N1 * Y1;
+      CP[(long)m * st] = cp;
+      DPv[(long)m * st] = dp;
+      psum = psum + cp;
+      palt = (m & 1) ? palt - cp : palt + cp;
+    }
+#undef XP
+  }
+  // BC rows for the velocity solves (DPTools.cpp:218-234)
+  const double fac1 = 2 * B.eta;
+  const double facK = fac1 * K;
+  cplx bc[6];
+  bc[0] = ((-1.0) * Dx) * psum / facK;
+  bc[1] = Dx * palt / facK;
+  bc[2] = ((-1.0) * Dy) * psum / facK;
+  bc[3] = Dy * palt / facK;
+  bc[4] = psum / fac1;
+  bc[5] = palt / fac1;
+
+  // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345) ----
+  cplx wall_bot[3], wall_top[3];
+  cplx* U = sc.U;
+  for (int c = 0; c < 3; ++c) {
+    for (int j = 0; j < Nz; ++j) {
+      cplx r;
+      if (c == 0) r = (Dx * CP[(long)j * st] - fin(fin_, 0, j)) / B.eta;
+      else if (c == 1) r = (Dy * CP[(long)j * st] - fin(fin_, 1, j)) / B.eta;
+      else r = (DPv[(long)j * st] - fin(fin_, 2, j)) / B.eta;
+      if (j >= 2) r = r - (DM2(j - 2) * U[(long)(j - 2) * st]) / beta[(long)(j - 1) * st];
+      U[(long)j * st] = r;
+    }
+    U[(long)(Nz - 1) * st] = U[(long)(Nz - 1) * st] / beta[(long)Nz * st];
+    U[(long)(Nz - 2) * st] = U[(long)(Nz - 2) * st] / beta[(long)(Nz - 1) * st];
+    for (int i = Nz - 3; i >= 0; --i)
+      U[(long)i * st] = (U[(long)i * st] - DP2(i) * U[(long)(i + 2) * st]) / beta[(long)(i + 1) * st];
+    cplx ya = mk(0, 0), yb = mk(0, 0);
+    for (int j = 0; j < Nz; ++j) {
+      const cplx x = U[(long)j * st];
+      ya = ya + x * C0(j);
+      yb = yb + x * C1(j);
+    }
+    ya = ya - bc[2 * c];
+    yb = yb - bc[2 * c + 1];
+    const cplx Ya = Gi00 * ya + Gi01 * yb;
+    const cplx Yb = Gi10 * ya + Gi11 * yb;
+#define XU(j) ((j) < Nz ? (U[(long)(j) * st] - ((((j) & 1) ? ab[(long)(j) * st] * Yb : ab[(long)(j) * st] * Ya))) \
+                        : ((j) == Nz ? Ya : Yb))
+    cplx sb = mk(0, 0), stp = mk(0, 0);
+    for (int m = 0; m < Nz; ++m) {
+      cplx cu = B.sd0[m] * XU(m);
+      if (m >= 2) cu = B.sm2[m] * XU(m - 2) + cu;
+      cu = cu + B.sp2[m] * XU(m + 2);
+      if (m == 0) cu = cu + B.S0N * Ya;
+      if (m == 1) cu = cu + B.S1N1 * Yb;
+      fout(c, m, cu);
+      stp = stp + cu;
+      sb = (m & 1) ? sb - cu : sb + cu;
+    }
+#undef XU
+    wall_bot[c] = sb;
+    wall_top[c] = stp;
+  }
+#undef D0
+#undef DP2
+#undef DM2
+#undef C0
+#undef C1
+
+  // ---- wall correction coefficients ----
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 4; ++b) res.corr[c][b] = mk(0, 0);
+  res.nbasis = 0;
+  if (B.geometry == MODE_BOTTOM) {
+    // DPTools.cpp:617-633; basis (e^{-Kz}, z e^{-Kz})
+    const cplx fx = -wall_bot[0], fy = -wall_bot[1], fz = -wall_bot[2];
+    const cplx alpha = kx * fx + ky * fy + times_i(K * fz);
+    res.nbasis = 2;
+    res.corr[0][0] = fx; res.corr[0][1] = ((-kx) * alpha) / K;
+    res.corr[1][0] = fy; res.corr[1][1] = ((-ky) * alpha) / K;
+    res.corr[2][0] = fz; res.corr[2][1] = -times_i(alpha);
+  } else if (B.geometry == MODE_SLIT) {
+    // DPTools.cpp:727-812; basis (e^{-kz}, e^{k(z-Hs)}, z e^{-kz}, z e^{k(z-Hs)})
+    const double k = sqrt(kx * kx + ky * ky);
+    const double Hs = B.Hs;
+    const double fac = 1.0 / 2.0 / B.eta;
+    const double enkh = exp(-k * Hs);
+    const cplx fbx = -wall_bot[0], fby = -wall_bot[1], fbz = -wall_bot[2];
+    const cplx ftx = -wall_top[0], fty = -wall_top[1], ftz = -wall_top[2];
+    cplx rhs[8];
+    rhs[0] = times_i(-kx * fbx) - times_i(ky * fby);
+    rhs[1] = times_i(-kx * ftx) - times_i(ky * fty);
+    rhs[2] = fbx; rhs[3] = fby; rhs[4] = fbz;
+    rhs[5] = ftx; rhs[6] = fty; rhs[7] = ftz;
+    // rows 0,1,4,7 of the 8x8 system couple only (x0, x1, x6, x7) and are
+    // real: solve that 4x4 block with partial pivoting, then the two 2x2
+    // blocks for (x2, x3) and (x4, x5).
+    double A[4][4] = {{fac, enkh * fac, -k, k * enkh},
+                      {(1 - k * Hs) * enkh * fac, (1 + k * Hs) * fac, -k * enkh, k},
+                      {0, 0, 1, enkh},
+                      {Hs * enkh * fac, Hs * fac, enkh, 1}};
+    cplx b4[4] = {rhs[0], rhs[1], rhs[4], rhs[7]};
+    for (int col = 0; col < 4; ++col) {
+      int piv = col;
+      double best = fabs(A[col][col]);
+      for (int r = col + 1; r < 4; ++r)
+        if (fabs(A[r][col]) > best) { best = fabs(A[r][col]); piv = r; }
+      if (piv != col) {
+        for (int cc = 0; cc < 4; ++cc) { double t = A[col][cc]; A[col][cc] = A[piv][cc]; A[piv][cc] = t; }
+        cplx t = b4[col]; b4[col] = b4[piv]; b4[piv] = t;
+      }
+      const double inv = 1.0 / A[col][col];
+      for (int r = col + 1; r < 4; ++r) {
+        const double l = A[r][col] * inv;
+        for (int cc = col + 1; cc < 4; ++cc) A[r][cc] -= l * A[col][cc];
+        b4[r] = b4[r] - l * b4[col];
+      }
+    }
+    cplx xs4[4];
+    for (int r = 3; r >= 0; --r) {
+      cplx s = b4[r];
+      for (int cc = r + 1; cc < 4; ++cc) s = s - A[r][cc] * xs4[cc];
+      xs4[r] = s / A[r][r];
+    }
+    const cplx X0 = xs4[0], X1 = xs4[1], X6 = xs4[2], X7 = xs4[3];
+    // rows 2,5: x2 + enkh x3 = b2 ; enkh x2 + x3 = b5 - (A50 X0 + A51 X1)
+    const double d2 = 1 - enkh * enkh;
+    const cplx r5 = rhs[5] - (times_i(-kx * Hs * enkh * fac / k * X0) + times_i(kx * Hs * fac / k * X1));
+    const cplx X2 = (rhs[2] - enkh * r5) / d2;
+    const cplx X3 = (r5 - enkh * rhs[2]) / d2;
+    const cplx r6 = rhs[6] - (times_i(-ky * Hs * enkh * fac / k * X0) + times_i(ky * Hs * fac / k * X1));
+    const cplx X4 = (rhs[3] - enkh * r6) / d2;
+    const cplx X5 = (r6 - enkh * rhs[3]) / d2;
+    res.nbasis = 4;
+    res.corr[0][0] = X2; res.corr[0][1] = X3;
+    res.corr[0][2] = times_i(-(kx * fac / k) * X0); res.corr[0][3] = times_i((kx * fac / k) * X1);
+    res.corr[1][0] = X4; res.corr[1][1] = X5;
+    res.corr[1][2] = times_i(-(ky * fac / k) * X0); res.corr[1][3] = times_i((ky * fac / k) * X1);
+    res.corr[2][0] = X6; res.corr[2][1] = X7;
+    res.corr[2][2] = fac * X0; res.corr[2][3] = fac * X1;
+  }
+}
+
+// k = 0 mode for the walled geometries (Solvers.py:482-516, 563-599): the
+// velocity BVP u'' = -f/eta with u(0)=0 and u'(top)=0 (bottom wall) or
+// u(0)=u(Lz)=0 (slit); w and the no-wall k=0 mode stay zero.
+template <class In, class Out>
+DPS_HD void bvp_k0(const BvpConsts& B, const In& fin_, Out& fout, cplx* work, long ws)
+{
+  const int Nz = B.Nz;
+  if (B.geometry != MODE_BOTTOM && B.geometry != MODE_SLIT) {
+    for (int c = 0; c < 3; ++c)
+      for (int m = 0; m < Nz; ++m) fout(c, m, mk(0, 0));
+    return;
+  }
+  const double* RA = (B.geometry == MODE_BOTTOM) ? B.R1 : B.R2;   // first BC row
+  const double* RB = B.L2;                                        // second BC row
+  // D = -BCs[:, Nz:]; Ginv = 1/det [[D11, -D01], [-D10, D00]]
+  const double D00 = -RA[Nz], D01 = -RA[Nz + 1], D10 = -RB[Nz], D11 = -RB[Nz + 1];
+  const double idet = 1.0 / (D00 * D11 - D01 * D10);
+  const double g00 = idet * D11, g01 = idet * (-D01), g10 = idet * (-D10), g11 = idet * D00;
+  for (int c = 0; c < 2; ++c) {
+    cplx ca = mk(0, 0), cb = mk(0, 0);
+    for (int j = 0; j < Nz; ++j) {
+      cplx f = fin(fin_, c, j);
+      if (B.geometry == MODE_SLIT) f = mk(f.re, f.re);   // Solvers.py:564-566
+      const cplx r = (-1.0) * f / B.eta;
+      work[(long)j * ws] = r;
+      ca = ca + RA[j] * r;
+      cb = cb + RB[j] * r;
+    }
+    const cplx a0 = g00 * ca + g01 * cb;
+    const cplx a1 = g10 * ca + g11 * cb;
+    for (int m = 0; m < Nz; ++m) {
+      cplx v = B.sd0[m] * work[(long)m * ws];
+      if (m >= 2) v = B.sm2[m] * work[(long)(m - 2) * ws] + v;
+      if (m + 2 < Nz) v = v + B.sp2[m] * work[(long)(m + 2) * ws];
+      else if (m + 2 == Nz) v = v + B.sp2[m] * a0;
+      else v = v + B.sp2[m] * a1;
+      if (m == 0) v = v + B.S0N * a0;
+      if (m == 1) v = v + B.S1N1 * a1;
+      fout(c, m, v);
+    }
+  }
+  for (int m = 0; m < Nz; ++m) fout(2, m, mk(0, 0));
+}
+
+}  // namespace dps

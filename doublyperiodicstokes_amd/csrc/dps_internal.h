@@ -1,0 +1,91 @@
+// dps_internal.h -- kernel argument blocks and launchers shared by the host
+// orchestration (dps_api.hip) and the kernels (dps_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dps_core.h"
+
+#define DPS_MAX_W 32   // largest x/y support (points) a stencil may have
+
+namespace dps {
+
+struct StencilArgs {
+  int N;
+  double xmin, ymin, zmin;
+  double Lx, Ly, Lz, hx, hy;
+  int Nx, Ny;
+  int has_dip;
+  KernelDesc km, kd;
+  const double *zext_m, *zw_m, *zext_d, *zw_d;
+  double* pos;     // 3N wrapped/shifted positions
+  int* err;        // set to 1 if a particle is outside [0, Lz]
+  int4 *sup_m, *sup_d;
+  void *rec_m, *rec_d;   // per-particle weight records (grid precision)
+  int rs_m, rs_d;        // record strides (elements)
+  int* key;              // tile index
+  int TX, TY, ntx;
+};
+
+struct SpreadArgs {
+  int Nx, Ny, Nz;
+  int TX, TY, ZC, ntx, nty, RX, RY;
+  const int *start, *order;
+  KernelDesc km, kd;
+  const int4 *sup_m, *sup_d;
+  const void *rec_m, *rec_d;
+  int rs_m, rs_d;
+  const int *kt_m, *kt_d;
+  const double *fA_m, *fB_m, *fA_d, *fB_d;
+  void* grid;   // [3][Nz][Ny][Nx] grid precision
+};
+
+struct InterpArgs {
+  int N, Nx, Ny, Nz;
+  const int* order;
+  KernelDesc km, kd;
+  const int4 *sup_m, *sup_d;
+  const void *rec_m, *rec_d;
+  int rs_m, rs_d;
+  const int *kt_m, *kt_d;
+  const double *fA_m, *fA_d, *fB_d;
+  const void* grid;
+};
+
+struct ZArgs {
+  int N, NH, NE, NO;   // nodes, node pairs, even/odd coefficient counts
+  int Nm, Nkx;         // modes per z level, kx count
+  const void *TfE, *TfO, *TiE, *TiO;   // grid precision
+  // wall correction (inverse only)
+  int nbasis;
+  const double2* corr;   // [Nm][3][4]
+  const double *kxs, *kys, *zpts;
+  double Hs;
+  double inv_nxy;   // 1/(Nx Ny): the c2r transform is unnormalised
+};
+
+struct BvpArgs {
+  BvpConsts B;   // pointers are device pointers
+  int Nx, Ny, Nkx, Nm;
+  double Lx, Ly;
+  const int* is_special;   // [Nm]
+  const int* special;      // list for the special launch (null for the main launch)
+  int nspecial;
+  double2* u2;             // [nspecial][3][Nz]
+  double *s_beta, *s_ab;   // [Nz+1][Nm], [Nz][Nm]
+  double2 *s_P, *s_CP, *s_DP, *s_U;   // [Nz][Nm] each
+  double2* corr;           // [Nm][3][4]
+};
+
+template <class Tin, class T> void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s);
+void launch_reset_err(int* e, hipStream_t s);
+void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N, int ntiles,
+                 hipStream_t s);
+template <class T, class Tin> void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s);
+template <class T, class Tout> void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s);
+size_t z_lds_bytes(const ZArgs& A, bool mixed);
+template <class T> void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s);
+template <class T> void launch_zinv(const ZArgs& A, const void* in, void* out, hipStream_t s);
+template <class T> void launch_bvp(const BvpArgs& A, const void* in, void* out, hipStream_t s);
+template <class T> void launch_ref_to_grid(const double* ref, void* g, int Nx, int Ny, int Nz, hipStream_t s);
+template <class T> void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipStream_t s);
+
+}  // namespace dps

@@ -1,0 +1,795 @@
+// dps_kernels.hip -- gfx950 kernels of the doubly periodic Stokes Mdot path.
+//
+//  k_stencil      wrap/shift positions, z-range check, per-particle support and
+//                 1-D ES weights (ParticleList.cpp:394-548), tile key
+//  k_tile_*       counting sort of particles by x-y tile
+//  k_spread       forces + torque curl onto the grid: one workgroup per
+//                 (x-y tile, z chunk), LDS accumulation, wall images folded on
+//                 the fly, one coalesced write per grid element
+//                 (SpreadInterp.cpp:302-448 + BoundaryConditions.h:22-361 +
+//                  SpreadInterpWrapper.cpp:66-107)
+//  k_interp       grid -> particles, 16 lanes per particle, derivative kernels
+//                 for the angular velocity (SpreadInterp.cpp:450-600,
+//                 BoundaryConditions.h:377-731, FCM.py:411-426)
+//  k_zfwd/k_zinv  Chebyshev transform in z on even/odd folded node pairs
+//                 (Transform.cpp:107-304); the inverse adds the analytic wall
+//                 correction profiles (DPTools.cpp:590-877) in value space
+//  k_bvp          per-wavenumber FP64 pressure + velocity solves, one mode per
+//                 lane (DPTools.cpp:81-345)
+//
+// Grid layout in HBM: real grid [c][z][y][x] (x fastest), spectra
+// [c][z or k][ky][kx] with kx in [0, Nx/2] (r2c half spectrum).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "dps_core.h"
+#include "dps_internal.h"
+
+namespace dps {
+
+template <class T> struct C2;
+template <> struct C2<float> { typedef float2 t; };
+template <> struct C2<double> { typedef double2 t; };
+
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// ---------------------------------------------------------------------------
+// positions -> stencils
+// ---------------------------------------------------------------------------
+template <class Tin, class T>
+__global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.N) return;
+  double x = (double)pos_in[3 * p] - A.xmin;
+  double y = (double)pos_in[3 * p + 1] - A.ymin;
+  double z = (double)pos_in[3 * p + 2] - A.zmin;
+  // periodic wrap in x, y (ParticleList.cpp:561-580)
+  if (x < 0) { int img = 1 - ((int)(x / A.Lx)); x += img * A.Lx; }
+  else if (x > A.Lx) { int img = (int)(x / A.Lx); x -= img * A.Lx; }
+  if (y < 0) { int img = 1 - ((int)(y / A.Ly)); y += img * A.Ly; }
+  else if (y > A.Ly) { int img = (int)(y / A.Ly); y -= img * A.Ly; }
+  if (!(z >= 0 && z <= A.Lz)) {
+    atomicOr(A.err, 1);
+    z = z < 0 ? 0.0 : A.Lz;   // keep the stencil in range; the call reports an error
+  }
+  A.pos[3 * p] = x;
+  A.pos[3 * p + 1] = y;
+  A.pos[3 * p + 2] = z;
+  const double hxhy = A.hx * A.hy;
+  for (int kind = 0; kind < (A.has_dip ? 2 : 1); ++kind) {
+    const KernelDesc& K = kind ? A.kd : A.km;
+    const double* zext = kind ? A.zext_d : A.zext_m;
+    const double* zwe = kind ? A.zw_d : A.zw_m;
+    Stencil s = stencil_support(x, y, z, A.hx, A.hy, K, zext);
+    if (s.nz > K.wzmax) s.nz = K.wzmax;   // cannot happen (wzmax bounds the window)
+    int4 sup;
+    sup.x = s.bx; sup.y = s.by; sup.z = s.e0; sup.w = s.nz;
+    (kind ? A.sup_d : A.sup_m)[p] = sup;
+    T* rec = (T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
+    double wx[DPS_MAX_W], dx[DPS_MAX_W], wy[DPS_MAX_W], dy[DPS_MAX_W];
+    stencil_weights_xy(x, A.hx, s.bx, K.wx, K, K.inv_norm3, wx, kind ? dx : nullptr);
+    stencil_weights_xy(y, A.hy, s.by, K.wy, K, 1.0, wy, kind ? dy : nullptr);
+    int o = 0;
+    for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)wx[i];
+    o += K.wx;
+    for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)wy[j];
+    o += K.wy;
+    if (kind) {
+      for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)dx[i];
+      o += K.wx;
+      for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
+      o += K.wy;
+    }
+    const double a2 = K.alpha * K.alpha;
+    const int W = K.wzmax;
+    for (int q = 0; q < s.nz; ++q) {
+      const int e = s.e0 + q;
+      const double v = clamp_off(zext[e] - z, K.thresh);
+      const double ph = es_phi(v, K.betaw, a2);
+      const double qw = hxhy * zwe[e];
+      rec[o + q] = (T)ph;                 // spread z weight
+      rec[o + W + q] = (T)(ph * qw);      // interpolation z weight
+      if (kind) {
+        const double d = ph * es_dfac(v, K.betaw, a2);
+        rec[o + 2 * W + q] = (T)d;
+        rec[o + 3 * W + q] = (T)(d * qw);
+      }
+    }
+    if (kind == 0) {
+      // tile key from the monopole support centre
+      const int cx = wrapi(s.bx + K.wx / 2, A.Nx), cy = wrapi(s.by + K.wy / 2, A.Ny);
+      A.key[p] = (cy / A.TY) * A.ntx + (cx / A.TX);
+    }
+  }
+}
+
+__global__ void k_reset_err(int* e) { *e = 0; }
+
+__global__ void k_fill_int(int* a, int n, int v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_tile_count(const int* __restrict__ key, int* counts, int* rank, int N)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) rank[p] = atomicAdd(&counts[key[p]], 1);
+}
+
+// exclusive scan of counts[0..n) into start[0..n], single workgroup of 1024
+__global__ void k_tile_scan(const int* __restrict__ counts, int* start, int n)
+{
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t > 0) ? part[t - 1] : 0;
+  for (int i = b; i < e; ++i) { start[i] = run; run += counts[i]; }
+  if (t == 1023) start[n] = part[1023];
+}
+
+__global__ void k_tile_scatter(const int* __restrict__ key, const int* __restrict__ rank,
+                               const int* __restrict__ start, int* order, int N)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < N) order[start[key[p]] + rank[p]] = p;
+}
+
+// ---------------------------------------------------------------------------
+// spread
+// ---------------------------------------------------------------------------
+// one contiguous piece of a periodic support range intersected with a tile:
+// support columns b..b+w-1 (unwrapped) against [t0, t0+n), images shifted by
+// multiples of the period.  Returns count, first support index in *i0 and the
+// tile-local column of that index in *l0.
+__device__ __forceinline__ int clip_piece(int b, int w, int t0, int n, int period, int shift,
+                                          int* i0, int* l0)
+{
+  const int lo = max(b + shift, t0), hi = min(b + shift + w, t0 + n);
+  if (hi <= lo) return 0;
+  *i0 = lo - (b + shift);
+  *l0 = lo - t0;
+  return hi - lo;
+}
+
+template <class T, class Tin>
+__global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restrict__ F,
+                                                const Tin* __restrict__ Tq)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
+  const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
+  const int X0 = tx * A.TX, Y0 = ty * A.TY, Z0 = zc * A.ZC;
+  const int nX = min(A.TX, A.Nx - X0), nY = min(A.TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
+  const int plane = A.TY * A.TX, vol = A.ZC * plane;
+  for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (T)0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  // neighbour tiles (unique after periodic wrap)
+  const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
+  const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
+  int job = 0;   // running particle counter used to deal particles to waves
+  for (int dyt = 0; dyt < nby; ++dyt) {
+    const int ty2 = wrapi(by0 + dyt, A.nty);
+    for (int dxt = 0; dxt < nbx; ++dxt) {
+      const int t2 = ty2 * A.ntx + wrapi(bx0 + dxt, A.ntx);
+      const int pb = A.start[t2], pe = A.start[t2 + 1];
+      for (int pi = pb; pi < pe; ++pi, ++job) {
+        if ((job % nwaves) != wave) continue;
+        const int p = A.order[pi];
+        for (int kind = 0; kind < (Tq ? 2 : 1); ++kind) {
+          const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
+          const KernelDesc& K = kind ? A.kd : A.km;
+          const int* kt = kind ? A.kt_d : A.kt_m;
+          const double* fA = kind ? A.fA_d : A.fA_m;
+          const double* fB = kind ? A.fB_d : A.fB_m;
+          const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
+          const T* wx = rec;
+          const T* wy = rec + K.wx;
+          const T* dxw = rec + K.wx + K.wy;
+          const T* dyw = dxw + K.wx;
+          const T* wz = kind ? (dyw + K.wy) : (rec + K.wx + K.wy);
+          const T* dzw = wz + 2 * K.wzmax;
+          T f0, f1, f2;
+          if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
+          else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
+          const int nzq = sup.w;
+          for (int sxp = -1; sxp <= 1; ++sxp) {
+            int ix0, lx0;
+            const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
+            if (!cx) continue;
+            for (int syp = -1; syp <= 1; ++syp) {
+              int iy0, ly0;
+              const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
+              if (!cy) continue;
+              const int tot = cx * cy * nzq;
+              for (int t = lane; t < tot; t += 64) {
+                const int a = t % cx;
+                const int r = t / cx;
+                const int b = r % cy;
+                const int q = r / cy;
+                const int e = sup.z + q;
+                const int k = kt[e] - Z0;
+                if (k < 0 || k >= nZ) continue;
+                const int i = ix0 + a, j = iy0 + b;
+                T* dst = acc + k * plane + (ly0 + b) * A.TX + (lx0 + a);
+                if (kind == 0) {
+                  const T fa = (T)fA[e];
+                  if (fa == (T)0) continue;
+                  const T w = wx[i] * wy[j] * wz[q] * fa;
+                  atomicAdd(dst, w * f0);
+                  atomicAdd(dst + vol, w * f1);
+                  atomicAdd(dst + 2 * vol, w * f2);
+                } else {
+                  const T fa = (T)fA[e], fb = (T)fB[e];
+                  if (fa == (T)0 && fb == (T)0) continue;
+                  const T gx = dxw[i] * wy[j] * wz[q] * fa;
+                  const T gy = wx[i] * dyw[j] * wz[q] * fa;
+                  const T gz = wx[i] * wy[j] * dzw[q] * fb;
+                  const T h = (T)0.5;
+                  atomicAdd(dst, h * (gy * f2 - gz * f1));
+                  atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
+                  atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // one write per grid element of the tile
+  const int rowsz = nX;
+  const int nrows = 3 * nZ * nY;
+  for (int t = threadIdx.x; t < nrows * A.TX; t += blockDim.x) {
+    const int ix = t % A.TX;
+    const int row = t / A.TX;
+    if (ix >= rowsz) continue;
+    const int jy = row % nY;
+    const int r2 = row / nY;
+    const int kz = r2 % nZ;
+    const int c = r2 / nZ;
+    grid[(((size_t)c * A.Nz + Z0 + kz) * A.Ny + Y0 + jy) * A.Nx + X0 + ix] =
+        acc[c * vol + kz * plane + jy * A.TX + ix];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// interpolation (16 lanes per particle)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T group_sum16(T v)
+{
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
+template <class T, class Tout>
+__global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__ vel,
+                                                Tout* __restrict__ ang)
+{
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if (g >= A.N) return;
+  const int p = A.order[g];
+  const T* __restrict__ U = reinterpret_cast<const T*>(A.grid);
+  const size_t cs = (size_t)A.Nz * A.Ny * A.Nx;
+  {
+    const int4 sup = A.sup_m[p];
+    const KernelDesc& K = A.km;
+    const T* rec = (const T*)A.rec_m + (size_t)p * A.rs_m;
+    const T* wx = rec;
+    const T* wy = rec + K.wx;
+    const T* wzi = rec + K.wx + K.wy + K.wzmax;
+    T u0 = 0, u1 = 0, u2 = 0;
+    const int tot = K.wx * K.wy * sup.w;
+    for (int t = l; t < tot; t += 16) {
+      const int a = t % K.wx;
+      const int r = t / K.wx;
+      const int b = r % K.wy;
+      const int q = r / K.wy;
+      const int e = sup.z + q;
+      const T fa = (T)A.fA_m[e];
+      if (fa == (T)0) continue;
+      const size_t idx = ((size_t)A.kt_m[e] * A.Ny + wrapi(sup.y + b, A.Ny)) * A.Nx + wrapi(sup.x + a, A.Nx);
+      const T w = wx[a] * wy[b] * wzi[q] * fa;
+      u0 += w * U[idx];
+      u1 += w * U[idx + cs];
+      u2 += w * U[idx + 2 * cs];
+    }
+    u0 = group_sum16(u0);
+    u1 = group_sum16(u1);
+    u2 = group_sum16(u2);
+    if (l == 0) {
+      vel[3 * p] = (Tout)u0;
+      vel[3 * p + 1] = (Tout)u1;
+      vel[3 * p + 2] = (Tout)u2;
+    }
+  }
+  if (!ang) return;
+  {
+    const int4 sup = A.sup_d[p];
+    const KernelDesc& K = A.kd;
+    const T* rec = (const T*)A.rec_d + (size_t)p * A.rs_d;
+    const T* wx = rec;
+    const T* wy = rec + K.wx;
+    const T* dxw = wy + K.wy;
+    const T* dyw = dxw + K.wx;
+    const T* wzi = dyw + K.wy + K.wzmax;
+    const T* dzi = dyw + K.wy + 3 * K.wzmax;
+    T d[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = 0;
+    const int tot = K.wx * K.wy * sup.w;
+    for (int t = l; t < tot; t += 16) {
+      const int a = t % K.wx;
+      const int r = t / K.wx;
+      const int b = r % K.wy;
+      const int q = r / K.wy;
+      const int e = sup.z + q;
+      const T fa = (T)A.fA_d[e], fb = (T)A.fB_d[e];
+      if (fa == (T)0 && fb == (T)0) continue;
+      const size_t idx = ((size_t)A.kt_d[e] * A.Ny + wrapi(sup.y + b, A.Ny)) * A.Nx + wrapi(sup.x + a, A.Nx);
+      const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
+      const T gx = dxw[a] * wy[b] * wzi[q] * fa;
+      const T gy = wx[a] * dyw[b] * wzi[q] * fa;
+      const T gz = wx[a] * wy[b] * dzi[q] * fb;
+      d[0] += gx * v0; d[1] += gx * v1; d[2] += gx * v2;
+      d[3] += gy * v0; d[4] += gy * v1; d[5] += gy * v2;
+      d[6] += gz * v0; d[7] += gz * v1; d[8] += gz * v2;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = group_sum16(d[i]);
+    if (l == 0) {
+      // omega = -1/2 (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)  (FCM.py:424-426)
+      ang[3 * p] = (Tout)((T)(-0.5) * (d[5] - d[7]));
+      ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[6] - d[2]));
+      ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[1] - d[3]));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev transform in z (folded even/odd node pairs)
+// Each workgroup: 64 complex columns (modes) of one component, all nodes.
+// ---------------------------------------------------------------------------
+template <class T> struct ZCols { enum { value = 64 }; };
+template <> struct ZCols<double> { enum { value = 32 }; };
+
+template <class T>
+__global__ void __launch_bounds__(256) k_zfwd(ZArgs A, const typename C2<T>::t* __restrict__ in,
+                                              typename C2<T>::t* __restrict__ out)
+{
+  typedef typename C2<T>::t V;
+  constexpr int CW = ZCols<T>::value, RS = 256 / CW;
+  extern __shared__ unsigned char smem_raw[];
+  V* E = reinterpret_cast<V*>(smem_raw);            // [NH][CW]
+  V* O = E + (size_t)A.NH * CW;                      // [NH][CW]
+  const int c = blockIdx.y;
+  const int m0 = blockIdx.x * CW;
+  const int col = threadIdx.x % CW;
+  const int m = m0 + col;
+  const int N = A.N;
+  const V* src = in + (size_t)c * N * A.Nm;
+  for (int j = threadIdx.x / CW; j < A.NH; j += RS) {
+    V a, b;
+    a.x = a.y = b.x = b.y = (T)0;
+    if (m < A.Nm) {
+      a = src[(size_t)j * A.Nm + m];
+      b = src[(size_t)(N - 1 - j) * A.Nm + m];
+    }
+    V e, o;
+    if (j == N - 1 - j) { e = a; o.x = o.y = (T)0; }
+    else { e.x = a.x + b.x; e.y = a.y + b.y; o.x = a.x - b.x; o.y = a.y - b.y; }
+    E[j * CW + col] = e;
+    O[j * CW + col] = o;
+  }
+  __syncthreads();
+  if (m >= A.Nm) return;
+  V* dst = out + (size_t)c * N * A.Nm;
+  for (int k = threadIdx.x / CW; k < N; k += RS) {
+    const int kk = k >> 1;
+    const T* Tr = (k & 1) ? ((const T*)A.TfO + (size_t)kk * A.NH) : ((const T*)A.TfE + (size_t)kk * A.NH);
+    const V* X = (k & 1) ? O : E;
+    T sr = 0, si = 0;
+    for (int j = 0; j < A.NH; ++j) {
+      const T t = Tr[j];
+      const V x = X[j * CW + col];
+      sr += t * x.x;
+      si += t * x.y;
+    }
+    V r;
+    r.x = sr;
+    r.y = si;
+    dst[(size_t)k * A.Nm + m] = r;
+  }
+}
+
+// inverse: values at nodes from coefficients, plus wall-correction profiles
+template <class T>
+__global__ void __launch_bounds__(256) k_zinv(ZArgs A, const typename C2<T>::t* __restrict__ in,
+                                              typename C2<T>::t* __restrict__ out)
+{
+  typedef typename C2<T>::t V;
+  constexpr int CW = ZCols<T>::value, RS = 256 / CW;
+  extern __shared__ unsigned char smem_raw[];
+  V* Ck = reinterpret_cast<V*>(smem_raw);   // [N][CW]
+  const int c = blockIdx.y;
+  const int m0 = blockIdx.x * CW;
+  const int col = threadIdx.x % CW;
+  const int m = m0 + col;
+  const int N = A.N;
+  const V* src = in + (size_t)c * N * A.Nm;
+  for (int k = threadIdx.x / CW; k < N; k += RS) {
+    V a;
+    a.x = a.y = (T)0;
+    if (m < A.Nm) a = src[(size_t)k * A.Nm + m];
+    Ck[k * CW + col] = a;
+  }
+  __syncthreads();
+  if (m >= A.Nm) return;
+  // wall correction coefficients of this mode/component
+  cplx cc[4];
+  int nb = A.nbasis;
+  double K = 0;
+  if (nb) {
+    const double2* cr = A.corr + ((size_t)m * 3 + c) * 4;
+    for (int b = 0; b < 4; ++b) { cc[b].re = cr[b].x; cc[b].im = cr[b].y; }
+    const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
+    const double kx = A.kxs[ix], ky = A.kys[iy];
+    K = sqrt(kx * kx + ky * ky);
+    if (m == 0) nb = 0;
+  }
+  V* dst = out + (size_t)c * N * A.Nm;
+  for (int j = threadIdx.x / CW; j < A.NH; j += RS) {
+    T ar = 0, ai = 0, br = 0, bi = 0;
+    const T* te = (const T*)A.TiE + (size_t)j * A.NE;
+    const T* to = (const T*)A.TiO + (size_t)j * A.NO;
+    for (int kk = 0; kk < A.NE; ++kk) {
+      const V x = Ck[(2 * kk) * CW + col];
+      ar += te[kk] * x.x;
+      ai += te[kk] * x.y;
+    }
+    for (int kk = 0; kk < A.NO; ++kk) {
+      const V x = Ck[(2 * kk + 1) * CW + col];
+      br += to[kk] * x.x;
+      bi += to[kk] * x.y;
+    }
+    const int j2 = N - 1 - j;
+    for (int side = 0; side < 2; ++side) {
+      const int jj = side ? j2 : j;
+      if (side && j2 == j) break;
+      T vr = side ? (ar - br) : (ar + br);
+      T vi = side ? (ai - bi) : (ai + bi);
+      if (nb) {
+        const double z = A.zpts[jj];
+        const double e1 = exp(-K * z);
+        double sr = cc[0].re * e1, si = cc[0].im * e1;
+        if (nb == 2) {
+          sr += cc[1].re * (z * e1);
+          si += cc[1].im * (z * e1);
+        } else {
+          const double e2 = exp(K * (z - A.Hs));
+          sr += cc[1].re * e2 + cc[2].re * (z * e1) + cc[3].re * (z * e2);
+          si += cc[1].im * e2 + cc[2].im * (z * e1) + cc[3].im * (z * e2);
+        }
+        vr += (T)(sr * A.inv_nxy);
+        vi += (T)(si * A.inv_nxy);
+      }
+      V r;
+      r.x = vr;
+      r.y = vi;
+      dst[(size_t)jj * A.Nm + m] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BVP: one mode per lane
+// ---------------------------------------------------------------------------
+template <class T>
+struct CoefIn {
+  const typename C2<T>::t* p;
+  long cs, ks;
+  bool conj;
+  DPS_HD cplx operator()(int c, int k) const
+  {
+    const typename C2<T>::t v = p[c * cs + k * ks];
+    return conj ? mk((double)v.x, -(double)v.y) : mk((double)v.x, (double)v.y);
+  }
+};
+template <class T>
+struct CoefOut {
+  typename C2<T>::t* p;
+  long cs, ks;
+  DPS_HD void operator()(int c, int k, cplx v)
+  {
+    typename C2<T>::t r;
+    r.x = (T)v.re;
+    r.y = (T)v.im;
+    p[c * cs + k * ks] = r;
+  }
+};
+struct CplxOut {
+  double2* p;
+  long cs, ks;
+  DPS_HD void operator()(int c, int k, cplx v)
+  {
+    double2 r;
+    r.x = v.re;
+    r.y = v.im;
+    p[c * cs + k * ks] = r;
+  }
+};
+
+__device__ __forceinline__ double kvec(int i, int n, double L)
+{
+  const int nn = (i <= n / 2) ? i : i - n;
+  return 2 * M_PI * (double)nn / L;
+}
+
+__device__ void store_corr(const BvpArgs& A, int m, const ModeResult& r)
+{
+  double2* cr = A.corr + (size_t)m * 12;
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 4; ++b) {
+      double2 v;
+      v.x = r.corr[c][b].re;
+      v.y = r.corr[c][b].im;
+      cr[c * 4 + b] = v;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_bvp(BvpArgs A, const typename C2<T>::t* __restrict__ fin,
+                                            typename C2<T>::t* __restrict__ fout)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int m;
+  if (A.special) {
+    if (t >= A.nspecial) return;
+    m = A.special[t];
+  } else {
+    m = t;
+    if (m >= A.Nm) return;
+    if (A.is_special[m]) return;
+  }
+  const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
+  CoefIn<T> in;
+  in.p = fin + m;
+  in.cs = (long)A.B.Nz * A.Nm;
+  in.ks = A.Nm;
+  in.conj = false;
+  CoefOut<T> out;
+  out.p = fout + m;
+  out.cs = in.cs;
+  out.ks = in.ks;
+  ScratchView sc;
+  sc.stride = A.Nm;
+  sc.beta = A.s_beta + m;
+  sc.ab = A.s_ab + m;
+  sc.P = reinterpret_cast<cplx*>(A.s_P) + m;
+  sc.CP = reinterpret_cast<cplx*>(A.s_CP) + m;
+  sc.DP = reinterpret_cast<cplx*>(A.s_DP) + m;
+  sc.U = reinterpret_cast<cplx*>(A.s_U) + m;
+  ModeResult r1;
+  if (m == 0) {
+    bvp_k0(A.B, in, out, sc.U, sc.stride);
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 4; ++b) r1.corr[c][b] = mk(0, 0);
+    store_corr(A, 0, r1);
+    return;
+  }
+  const double kx = kvec(ix, A.Nx, A.Lx), ky = kvec(iy, A.Ny, A.Ly);
+  const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
+  const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+  bvp_mode(A.B, kx, ky, nyqx, nyqy, in, out, sc, r1);
+  if (!A.special) {
+    store_corr(A, m, r1);
+    return;
+  }
+  // mode touching a Nyquist index: the reference's c2c transform keeps only
+  // the real part of the inverse, i.e. the Hermitian average of this mode and
+  // its mirror solved with the mirror's own (grid) wavenumbers.
+  const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
+  const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
+  const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
+  const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
+  in.conj = true;
+  CplxOut o2;
+  o2.p = A.u2 + (size_t)t * 3 * A.B.Nz;
+  o2.cs = A.B.Nz;
+  o2.ks = 1;
+  ModeResult r2;
+  bvp_mode(A.B, kxm, kym, nyqxm, nyqym, in, o2, sc, r2);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < A.B.Nz; ++k) {
+      typename C2<T>::t* q = fout + (size_t)c * in.cs + (size_t)k * in.ks + m;
+      const double2 v2 = o2.p[c * o2.cs + k];
+      const double re = 0.5 * ((double)q->x + v2.x);
+      const double im = 0.5 * ((double)q->y - v2.y);
+      q->x = (T)re;
+      q->y = (T)im;
+    }
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 4; ++b)
+      r1.corr[c][b] = mk(0.5 * (r1.corr[c][b].re + r2.corr[c][b].re),
+                         0.5 * (r1.corr[c][b].im - r2.corr[c][b].im));
+  store_corr(A, m, r1);
+}
+
+// ---------------------------------------------------------------------------
+// layout conversions for the stage entry points:
+//   reference interior layout (Ny, Nx, Nz, 3) FP64  <->  [c][z][y][x] T
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_ref_to_grid(const double* __restrict__ ref, T* __restrict__ g, int Nx, int Ny, int Nz)
+{
+  const size_t n = (size_t)3 * Nx * Ny * Nz;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = i % Nx;
+  size_t r = i / Nx;
+  const int y = r % Ny;
+  r /= Ny;
+  const int z = r % Nz;
+  const int c = r / Nz;
+  g[i] = (T)ref[(((size_t)y * Nx + x) * Nz + z) * 3 + c];
+}
+
+template <class T>
+__global__ void k_grid_to_ref(const T* __restrict__ g, double* __restrict__ ref, int Nx, int Ny, int Nz)
+{
+  const size_t n = (size_t)3 * Nx * Ny * Nz;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = i % Nx;
+  size_t r = i / Nx;
+  const int y = r % Ny;
+  r /= Ny;
+  const int z = r % Nz;
+  const int c = r / Nz;
+  ref[(((size_t)y * Nx + x) * Nz + z) * 3 + c] = (double)g[i];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class Tin, class T>
+void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s)
+{
+  if (A.N == 0) return;
+  hipLaunchKernelGGL((k_stencil<Tin, T>), dim3((A.N + 127) / 128), dim3(128), 0, s, A,
+                     (const Tin*)pos);
+}
+
+void launch_reset_err(int* e, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset_err, dim3(1), dim3(1), 0, s, e);
+}
+
+void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N, int ntiles,
+                 hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fill_int, dim3((ntiles + 255) / 256), dim3(256), 0, s, counts, ntiles, 0);
+  if (N > 0)
+    hipLaunchKernelGGL(k_tile_count, dim3((N + 255) / 256), dim3(256), 0, s, key, counts, rank, N);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, counts, start, ntiles);
+  if (N > 0)
+    hipLaunchKernelGGL(k_tile_scatter, dim3((N + 255) / 256), dim3(256), 0, s, key, rank, start,
+                       order, N);
+}
+
+template <class T, class Tin>
+void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s)
+{
+  const size_t lds = (size_t)3 * A.ZC * A.TY * A.TX * sizeof(T);
+  dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
+  hipLaunchKernelGGL((k_spread<T, Tin>), grid, dim3(256), lds, s, A, (const Tin*)F,
+                     (const Tin*)Tq);
+}
+
+template <class T, class Tout>
+void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s)
+{
+  if (A.N == 0) return;
+  const int groups_per_block = 256 / 16;
+  hipLaunchKernelGGL((k_interp<T, Tout>), dim3((A.N + groups_per_block - 1) / groups_per_block),
+                     dim3(256), 0, s, A, (Tout*)vel, (Tout*)ang);
+}
+
+template <class T>
+size_t zfwd_lds(const ZArgs& A) { return (size_t)2 * A.NH * ZCols<T>::value * sizeof(typename C2<T>::t); }
+template <class T>
+size_t zinv_lds(const ZArgs& A) { return (size_t)A.N * ZCols<T>::value * sizeof(typename C2<T>::t); }
+
+template <class T>
+void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  constexpr int CW = ZCols<T>::value;
+  const size_t lds = zfwd_lds<T>(A);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_zfwd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_zfwd<T>), dim3((A.Nm + CW - 1) / CW, 3), dim3(256), lds, s, A,
+                     (const V*)in, (V*)out);
+}
+
+template <class T>
+void launch_zinv(const ZArgs& A, const void* in, void* out, hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  constexpr int CW = ZCols<T>::value;
+  const size_t lds = zinv_lds<T>(A);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_zinv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_zinv<T>), dim3((A.Nm + CW - 1) / CW, 3), dim3(256), lds, s, A,
+                     (const V*)in, (V*)out);
+}
+
+size_t z_lds_bytes(const ZArgs& A, bool mixed)
+{
+  return mixed ? std::max(zfwd_lds<float>(A), zinv_lds<float>(A))
+               : std::max(zfwd_lds<double>(A), zinv_lds<double>(A));
+}
+
+template <class T>
+void launch_bvp(const BvpArgs& A, const void* in, void* out, hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  const int n = A.special ? A.nspecial : A.Nm;
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_bvp<T>), dim3((n + 63) / 64), dim3(64), 0, s, A, (const V*)in, (V*)out);
+}
+
+template <class T>
+void launch_ref_to_grid(const double* ref, void* g, int Nx, int Ny, int Nz, hipStream_t s)
+{
+  const size_t n = (size_t)3 * Nx * Ny * Nz;
+  hipLaunchKernelGGL((k_ref_to_grid<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ref,
+                     (T*)g, Nx, Ny, Nz);
+}
+
+template <class T>
+void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipStream_t s)
+{
+  const size_t n = (size_t)3 * Nx * Ny * Nz;
+  hipLaunchKernelGGL((k_grid_to_ref<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (const T*)g, ref, Nx, Ny, Nz);
+}
+
+#define DPS_INST(T)                                                                          \
+  template void launch_stencil<float, T>(const StencilArgs&, const void*, hipStream_t);     \
+  template void launch_stencil<double, T>(const StencilArgs&, const void*, hipStream_t);    \
+  template void launch_spread<T, float>(const SpreadArgs&, const void*, const void*, hipStream_t); \
+  template void launch_spread<T, double>(const SpreadArgs&, const void*, const void*, hipStream_t); \
+  template void launch_interp<T, float>(const InterpArgs&, void*, void*, hipStream_t);      \
+  template void launch_interp<T, double>(const InterpArgs&, void*, void*, hipStream_t);     \
+  template void launch_zfwd<T>(const ZArgs&, const void*, void*, hipStream_t);              \
+  template void launch_zinv<T>(const ZArgs&, const void*, void*, hipStream_t);              \
+  template void launch_bvp<T>(const BvpArgs&, const void*, void*, hipStream_t);             \
+  template void launch_ref_to_grid<T>(const double*, void*, int, int, int, hipStream_t);    \
+  template void launch_grid_to_ref<T>(const void*, double*, int, int, int, hipStream_t);
+
+DPS_INST(float)
+DPS_INST(double)
+
+}  // namespace dps

@@ -1,0 +1,296 @@
+"""The GPU mobility solver behind the joint interface.
+
+``StokesParameters`` and ``DPStokes`` mirror the reference's pybind module
+``uammd`` (``source/gpu/python_wrapper/uammd_python.cpp:59-129``) so that a
+caller of ``uammd.DPStokes`` can switch to this module:
+
+    par = StokesParameters(viscosity=..., Lx=..., Ly=..., zmin=..., zmax=...,
+                           w=6, w_d=6, beta=10.71, beta_d=10.71,
+                           nx=256, ny=256, nz=96, mode='slit')
+    solver = DPStokes()
+    solver.initialize(par, numberParticles)
+    solver.setPositions(positions)                       # [x0,y0,z0,x1,...]
+    solver.Mdot(forces=F, torques=T, velocities=MF, angularVelocities=MT)
+    solver.clear()
+
+Differences from the reference module, all deliberate:
+
+* ``precision`` ('double' or 'mixed') is chosen per solver instead of at
+  compile time; ``getPrecision()`` reports the default ('double', or
+  'single' when ``DPSTOKES_PRECISION=mixed``).
+* ``alpha``/``alpha_d`` are physical support half-widths and default to
+  ``w*h/2`` -- the CPU reference's value (``ParticleList.cpp:229``).
+* The torque kernel is the derivative ES kernel of the CPU reference
+  (``w_d``, ``beta_d``, ``alpha_d``); ``w_d = 0`` disables torques.
+* Output arrays of the wrong dtype raise ``TypeError`` instead of being
+  silently ignored.
+* Besides numpy arrays, every method also accepts device-resident torch
+  tensors; ``mdot_device`` returns device tensors without host copies.
+
+All arithmetic happens in the HIP library; if it is not built this module
+raises on use.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _native as N
+
+_MODES = {"nowall": N.DPS_MODE_NOWALL, "bottom": N.DPS_MODE_BOTTOM, "slit": N.DPS_MODE_SLIT}
+
+
+def getPrecision():
+    """Default grid precision of new solvers: 'double' or 'single' (mixed)."""
+    return "single" if os.environ.get("DPSTOKES_PRECISION", "double") == "mixed" else "double"
+
+
+class StokesParameters:
+    """Parameters of a doubly periodic Stokes solve (uammd.StokesParameters).
+
+    ``beta``/``beta_d`` are the ES exponents beta*w; ``<= 0`` selects 1.785*w.
+    ``mode`` is 'nowall', 'bottom' or 'slit'.
+    """
+
+    def __init__(self, viscosity=1.0, Lx=0.0, Ly=0.0, zmin=0.0, zmax=0.0, w=1, w_d=1,
+                 alpha=-1.0, alpha_d=-1.0, beta=-1.0, beta_d=-1.0, nx=-1, ny=-1, nz=-1,
+                 mode="none", precision=None):
+        self.viscosity = viscosity
+        self.Lx, self.Ly = Lx, Ly
+        self.zmin, self.zmax = zmin, zmax
+        self.w, self.w_d = w, w_d
+        self.alpha, self.alpha_d = alpha, alpha_d
+        self.beta, self.beta_d = beta, beta_d
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.mode = mode
+        self.precision = precision if precision is not None else (
+            "mixed" if getPrecision() == "single" else "double")
+
+    def __str__(self):
+        return ("viscosity: %g\nLx: %g\nLy: %g\nzmin: %g\nzmax: %g\nw: %g\nw_d: %g\nalpha: %g\n"
+                "alpha_d: %g\nbeta: %g\nbeta_d: %g\nnx: %d\nny: %d\nnz: %d\nmode: %s\nprecision: %s\n"
+                % (self.viscosity, self.Lx, self.Ly, self.zmin, self.zmax, self.w, self.w_d,
+                   self.alpha, self.alpha_d, self.beta, self.beta_d, self.nx, self.ny, self.nz,
+                   self.mode, self.precision))
+
+    def to_c(self):
+        if self.mode not in _MODES:
+            raise ValueError("mode must be one of 'nowall', 'bottom', 'slit' (got %r)" % (self.mode,))
+        if self.precision not in ("double", "mixed"):
+            raise ValueError("precision must be 'double' or 'mixed'")
+        p = N.DpsParams()
+        p.viscosity = float(self.viscosity)
+        p.Lx, p.Ly = float(self.Lx), float(self.Ly)
+        p.zmin, p.zmax = float(self.zmin), float(self.zmax)
+        p.w, p.w_d = int(self.w), int(self.w_d)
+        p.alpha, p.alpha_d = float(self.alpha), float(self.alpha_d)
+        p.beta, p.beta_d = float(self.beta), float(self.beta_d)
+        p.nx, p.ny, p.nz = int(self.nx), int(self.ny), int(self.nz)
+        p.mode = _MODES[self.mode]
+        p.precision = N.DPS_PREC_MIXED if self.precision == "mixed" else N.DPS_PREC_DOUBLE
+        return p
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dtype_code(dt):
+    torch = _torch()
+    if dt in (np.float32, torch.float32):
+        return N.DPS_F32
+    if dt in (np.float64, torch.float64):
+        return N.DPS_F64
+    raise TypeError("arrays must be float32 or float64 (got %s)" % (dt,))
+
+
+class DPStokes:
+    """Doubly periodic Stokes mobility on one MI355X (uammd.DPStokes)."""
+
+    def __init__(self):
+        self._h = None
+        self.N = 0
+        self.par = None
+
+    # -- lifetime -------------------------------------------------------------
+    def initialize(self, par, numberParticles):
+        """Create (or re-create) the solver for ``numberParticles`` particles."""
+        L = N.load()
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("DPStokes needs a HIP device")
+        self.clear()
+        h = ctypes.c_void_p()
+        cpar = par.to_c()
+        N.check(L.dps_create(ctypes.byref(cpar), int(numberParticles), ctypes.byref(h)))
+        self._h = h
+        self.N = int(numberParticles)
+        self.par = par
+        self.has_torque = int(par.w_d) > 0
+        self.dtype = torch.float32 if par.precision == "mixed" else torch.float64
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def clear(self):
+        """Release device memory; the solver must be initialised again to be used."""
+        if self._h is not None:
+            N.load().dps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.clear()
+        except Exception:
+            pass
+
+    def _need(self):
+        if self._h is None:
+            raise RuntimeError("DPStokes is not initialized. Call Initialize first")
+
+    # -- data movement --------------------------------------------------------
+    def _to_device(self, a, name, size):
+        torch = _torch()
+        if isinstance(a, torch.Tensor):
+            t = a.detach()
+            if t.device.type != "cuda":
+                t = t.to(self.device)
+        else:
+            arr = np.asarray(a)
+            if arr.dtype not in (np.float32, np.float64):
+                arr = arr.astype(np.float64)
+            t = torch.from_numpy(np.ascontiguousarray(arr)).to(self.device)
+        t = t.reshape(-1).contiguous()
+        if t.numel() != size:
+            raise ValueError("%s must hold %d values (got %d)" % (name, size, t.numel()))
+        if t.dtype not in (torch.float32, torch.float64):
+            t = t.to(torch.float64)
+        return t
+
+    def setPositions(self, positions):
+        """Positions [x0,y0,z0,...]; x, y wrap periodically, z must be in [zmin, zmax]."""
+        self._need()
+        p = self._to_device(positions, "positions", 3 * self.N)
+        N.check(N.load().dps_set_positions(self._h, ctypes.c_void_p(p.data_ptr()),
+                                           _dtype_code(p.dtype), _stream()))
+
+    def mdot_device(self, forces, torques=None, out_dtype=None):
+        """Mdot on device tensors; returns (velocities, angular velocities or None)."""
+        self._need()
+        torch = _torch()
+        F = self._to_device(forces, "forces", 3 * self.N)
+        dt = out_dtype if out_dtype is not None else F.dtype
+        F = F.to(dt)
+        T = None
+        if torques is not None and self.has_torque:
+            T = self._to_device(torques, "torques", 3 * self.N)
+            if T.numel() == 0:
+                T = None
+            else:
+                T = T.to(dt)
+        U = torch.empty(3 * self.N, dtype=dt, device=self.device)
+        W = torch.empty(3 * self.N, dtype=dt, device=self.device) if T is not None else None
+        N.check(N.load().dps_mdot(self._h, ctypes.c_void_p(F.data_ptr()),
+                                  ctypes.c_void_p(T.data_ptr()) if T is not None else None,
+                                  ctypes.c_void_p(U.data_ptr()),
+                                  ctypes.c_void_p(W.data_ptr()) if W is not None else None,
+                                  _dtype_code(dt), _stream()))
+        return U, W
+
+    def Mdot(self, forces, torques=None, velocities=None, angularVelocities=None):
+        """Velocities (and angular velocities when torques are given).
+
+        As in the reference, ``velocities``/``angularVelocities`` are output
+        arrays written in place; if they are omitted new numpy arrays are
+        returned.  Torques of size < 3N are treated as absent.
+        """
+        self._need()
+        tq = None
+        if torques is not None and np.size(torques) >= 3 * self.N and self.N > 0:
+            tq = torques
+        out_np = np.float64 if self.par.precision == "double" else np.float32
+        if isinstance(velocities, np.ndarray):
+            out_np = velocities.dtype.type
+        torch = _torch()
+        dt = torch.float32 if out_np == np.float32 else torch.float64
+        U, W = self.mdot_device(forces, tq, out_dtype=dt)
+        Uh = U.cpu().numpy()
+        Wh = W.cpu().numpy() if W is not None else np.zeros(0, dtype=out_np)
+        if velocities is not None:
+            if not isinstance(velocities, np.ndarray) or velocities.dtype != Uh.dtype:
+                raise TypeError("velocities must be a numpy array of dtype %s" % Uh.dtype)
+            velocities.reshape(-1)[:] = Uh
+        else:
+            velocities = Uh
+        if W is not None:
+            if angularVelocities is not None:
+                if not isinstance(angularVelocities, np.ndarray) or angularVelocities.dtype != Wh.dtype:
+                    raise TypeError("angularVelocities must be a numpy array of dtype %s" % Wh.dtype)
+                angularVelocities.reshape(-1)[:] = Wh
+            else:
+                angularVelocities = Wh
+        elif angularVelocities is None:
+            angularVelocities = Wh
+        return velocities, angularVelocities
+
+    # -- stages (parity testing, adjoint building blocks) --------------------
+    def info(self):
+        self._need()
+        out = (ctypes.c_int * 8)()
+        N.check(N.load().dps_get_info(self._h, out))
+        keys = ("nx", "ny", "nz", "N", "nzeff", "nzeff_d", "wzmax", "wzmax_d")
+        return dict(zip(keys, list(out)))
+
+    def stage_spread(self, forces, torques=None):
+        """Spread forces (+ torque curl); returns the grid (Ny, Nx, Nz, 3) FP64 on device."""
+        self._need()
+        torch = _torch()
+        i = self.info()
+        F = self._to_device(forces, "forces", 3 * self.N)
+        T = None
+        if torques is not None and self.has_torque:
+            T = self._to_device(torques, "torques", 3 * self.N).to(F.dtype)
+        G = torch.empty((i["ny"], i["nx"], i["nz"], 3), dtype=torch.float64, device=self.device)
+        N.check(N.load().dps_stage_spread(self._h, ctypes.c_void_p(F.data_ptr()),
+                                          ctypes.c_void_p(T.data_ptr()) if T is not None else None,
+                                          _dtype_code(F.dtype), ctypes.c_void_p(G.data_ptr()), _stream()))
+        return G
+
+    def stage_solve(self, grid):
+        """Velocity grid from a force grid, both (Ny, Nx, Nz, 3) FP64."""
+        self._need()
+        torch = _torch()
+        G = grid.to(self.device, torch.float64).contiguous()
+        out = torch.empty_like(G)
+        N.check(N.load().dps_stage_solve(self._h, ctypes.c_void_p(G.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), _stream()))
+        return out
+
+    def stage_interp(self, grid, with_angular=False, dtype=None):
+        """Interpolate a (Ny, Nx, Nz, 3) FP64 velocity grid to the particles."""
+        self._need()
+        torch = _torch()
+        G = grid.to(self.device, torch.float64).contiguous()
+        dt = dtype or torch.float64
+        U = torch.empty(3 * self.N, dtype=dt, device=self.device)
+        W = torch.empty(3 * self.N, dtype=dt, device=self.device) if with_angular and self.has_torque else None
+        N.check(N.load().dps_stage_interp(self._h, ctypes.c_void_p(G.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                                          ctypes.c_void_p(W.data_ptr()) if W is not None else None,
+                                          _dtype_code(dt), _stream()))
+        return U, W
+
+    def set_profiling(self, enable=True):
+        self._need()
+        N.check(N.load().dps_set_profiling(self._h, int(bool(enable))))
+
+    def timings(self):
+        """Per-stage microseconds of the last profiled Mdot (HIP events)."""
+        self._need()
+        out = (ctypes.c_double * 8)()
+        N.check(N.load().dps_get_timings(self._h, out))
+        keys = ("locate", "spread", "fft_fwd", "z_fwd", "bvp", "z_inv", "fft_inv", "interp")
+        return dict(zip(keys, list(out)))

@@ -1,0 +1,127 @@
+/*
+ * dpstokes_hip.h -- C ABI of the MI355X (gfx950) doubly periodic Stokes
+ * mobility solver.
+ *
+ * The entry points replace the GPU backend that the reference's joint
+ * interface binds:
+ *   reference pybind module `uammd` (source/gpu/python_wrapper/uammd_python.cpp:59-129)
+ *   -> C++ glue DPStokesGlue        (source/gpu/python_wrapper/uammd_interface.h:43-65)
+ *   -> DPStokesUAMMD                (source/gpu/python_wrapper/uammd_wrapper.cu:90-177)
+ * and the CPU backend's FCM class (source/cpu/python/FCM.py:149-441) it sits
+ * beside.  Unlike the reference glue (std::string, shared_ptr, exceptions)
+ * this is a plain C ABI: POD structs, device pointers, sizes and an int
+ * status; no exception or exit() crosses it.
+ *
+ * Conventions
+ *   - All particle arrays are interleaved [x0,y0,z0,x1,...] device pointers
+ *     (hipMalloc'ed or torch-owned), of the dtype named by `dtype`.
+ *   - `stream` is a hipStream_t passed as void*; 0 means the null stream.
+ *     Every call is asynchronous on that stream unless stated otherwise.
+ *   - Positions: x and y are periodic (wrapped into [0,L) as the reference's
+ *     ParticleList::update does, src/ParticleList.cpp:561-580); z must lie in
+ *     [zmin, zmax] or DPS_ERR_RANGE is reported by dps_set_positions.
+ */
+#ifndef DPSTOKES_HIP_H
+#define DPSTOKES_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DPS_OK 0
+#define DPS_ERR_ARG (-1)        /* invalid argument / parameter */
+#define DPS_ERR_STATE (-2)      /* handle not initialised, positions not set */
+#define DPS_ERR_RANGE (-3)      /* a particle is outside the z extent */
+#define DPS_ERR_HIP (-4)        /* HIP runtime error */
+#define DPS_ERR_FFT (-5)        /* rocFFT error */
+#define DPS_ERR_ALLOC (-6)      /* device allocation failed */
+
+/* geometry; mode strings of uammd_wrapper.cu:55-66 ("periodic" is not on this path) */
+#define DPS_MODE_NOWALL 1
+#define DPS_MODE_BOTTOM 2
+#define DPS_MODE_SLIT 3
+
+/* arithmetic */
+#define DPS_PREC_DOUBLE 0       /* everything in FP64 */
+#define DPS_PREC_MIXED 1        /* FP32 grids/FFT/z-transform, FP64 BVP + corrections */
+
+/* dtype of user arrays */
+#define DPS_F32 0
+#define DPS_F64 1
+
+/*
+ * Parameters; fields and defaults mirror uammd.StokesParameters
+ * (uammd_python.cpp:74-127, uammd_interface.h:20-41).  Lengths are physical.
+ *   beta, beta_d : ES exponent beta*w (uammd convention); <=0 selects 1.785*w.
+ *   alpha, alpha_d : kernel support half-width; <=0 selects w*h/2 (the CPU
+ *                    reference's value, src/ParticleList.cpp:229).
+ *   w_d/beta_d/alpha_d describe the derivative (dipole) kernel used for torques.
+ */
+typedef struct dps_params {
+  double viscosity;
+  double Lx, Ly;
+  double zmin, zmax;
+  int w, w_d;
+  double alpha, alpha_d;
+  double beta, beta_d;
+  int nx, ny, nz;
+  int mode;
+  int precision;
+} dps_params;
+
+typedef struct dps_handle dps_handle;
+
+/* Create a solver for `number_particles` particles (DPStokesGlue::initialize,
+   uammd_wrapper.cu:119-133).  Allocates all device memory and FFT plans on the
+   current HIP device.  *out is NULL on failure. */
+int dps_create(const dps_params* params, int number_particles, dps_handle** out);
+
+/* Release everything (DPStokesGlue::clear).  NULL is accepted. */
+int dps_destroy(dps_handle* h);
+
+/* Set particle positions from a device array of 3N values (DPStokesGlue::
+   setPositions, uammd_wrapper.cu:136-143; FCM.SetPositions, FCM.py:270-332).
+   Synchronises `stream` to report DPS_ERR_RANGE. */
+int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* stream);
+
+/* Mobility product (DPStokesGlue::Mdot, uammd_wrapper.cu:146-170; FCM.Mdot,
+   FCM.py:334-441): velocities = M_tt F + M_tr T, angular = M_rt F + M_rr T.
+   torques and angular may be NULL (then no torque is applied and angular
+   velocities are not computed), matching the reference's `torques.size < 3N`
+   rule (uammd_python.cpp:51-53).  All four arrays use `dtype`. */
+int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* velocities,
+             void* angular, int dtype, void* stream);
+
+/* Stage entry points (for stage-level parity tests and the adjoint):
+   spread forces (+torque curl) to the real grid, returned in the reference's
+   interior layout (Ny, Nx, Nz, 3) FP64; solve grid -> velocity grid (same
+   layout, FP64); interpolate a velocity grid (same layout) to particles. */
+int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int dtype,
+                     double* grid_out, void* stream);
+int dps_stage_solve(dps_handle* h, const double* grid_in, double* grid_out, void* stream);
+int dps_stage_interp(dps_handle* h, const double* grid_in, void* velocities, void* angular,
+                     int dtype, void* stream);
+
+/* Query derived sizes: nx, ny, nz, number of particles, Nzeff of the monopole
+   and dipole z-extended grids, z support widths.  `out` holds 8 ints. */
+int dps_get_info(const dps_handle* h, int* out);
+
+/* Per-kernel timings of the last dps_mdot in microseconds (HIP events on the
+   call's stream, enabled by dps_set_profiling).  `out` holds 8 doubles:
+   locate+sort, spread, fft forward, z forward, bvp, z inverse, fft inverse,
+   interp. Returns DPS_ERR_STATE if profiling is off. */
+int dps_set_profiling(dps_handle* h, int enable);
+int dps_get_timings(dps_handle* h, double* out);
+
+/* "single" or "double" for the handle's grid arithmetic (uammd getPrecision,
+   uammd_wrapper.cu:214-221).  Never NULL. */
+const char* dps_get_precision(const dps_handle* h);
+
+/* Last error message of this thread (static storage). */
+const char* dps_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPSTOKES_HIP_H */

@@ -1,0 +1,138 @@
+// Host-side harness over the solver's shared core (dps_core.h / dps_tables.h):
+// runs the exact per-mode BVP, stencil and table code that the gfx950 kernels
+// run, on the CPU, so tests can compare it with the FP64 oracle without a GPU.
+// Test infrastructure only (built by tests/conftest.py with g++).
+#include <cstring>
+#include <vector>
+#include "dps_core.h"
+#include "dps_tables.h"
+
+using namespace dps;
+
+namespace {
+struct HostIn {
+  const double* p;   // [3][Nz][2]
+  int Nz;
+  bool conj;
+  cplx operator()(int c, int k) const
+  {
+    const double* q = p + 2 * (c * Nz + k);
+    return conj ? mk(q[0], -q[1]) : mk(q[0], q[1]);
+  }
+};
+struct HostOut {
+  double* p;
+  int Nz;
+  void operator()(int c, int k, cplx v)
+  {
+    double* q = p + 2 * (c * Nz + k);
+    q[0] = v.re;
+    q[1] = v.im;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+// Chebyshev bands / BC rows: out arrays of length Nz (bands) and Nz+2 (rows)
+void cc_cheb_tables(int Nz, double H, double* sd0, double* sm2, double* sp2, double* fm1, double* fp1,
+                    double* scal /*S0N,S1N1,F0N1*/, double* R1, double* R2, double* L1, double* L2)
+{
+  ChebTables T = build_cheb_tables(Nz, H);
+  std::memcpy(sd0, T.sd0.data(), Nz * 8);
+  std::memcpy(sm2, T.sm2.data(), Nz * 8);
+  std::memcpy(sp2, T.sp2.data(), Nz * 8);
+  std::memcpy(fm1, T.fm1.data(), Nz * 8);
+  std::memcpy(fp1, T.fp1.data(), Nz * 8);
+  scal[0] = T.S0N; scal[1] = T.S1N1; scal[2] = T.F0N1;
+  std::memcpy(R1, T.R1.data(), (Nz + 2) * 8);
+  std::memcpy(R2, T.R2.data(), (Nz + 2) * 8);
+  std::memcpy(L1, T.L1.data(), (Nz + 2) * 8);
+  std::memcpy(L2, T.L2.data(), (Nz + 2) * 8);
+}
+
+// dense forward/inverse Chebyshev matrices rebuilt from the folded tables
+void cc_cheb_transform(int N, double* Tf /*N*N*/, double* Ti /*N*N*/)
+{
+  ChebTransform C = build_cheb_transform(N);
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j) {
+      const int jj = (j < C.NH) ? j : N - 1 - j;
+      const double sgn = (j < C.NH) ? 1.0 : ((k % 2) ? -1.0 : 1.0);
+      const double f = (k % 2) ? C.TfO[(size_t)(k / 2) * C.NH + jj] : C.TfE[(size_t)(k / 2) * C.NH + jj];
+      const double i = (k % 2) ? C.TiO[(size_t)jj * C.NO + k / 2] : C.TiE[(size_t)jj * C.NE + k / 2];
+      Tf[k * N + j] = sgn * f;
+      Ti[j * N + k] = sgn * i;
+    }
+}
+
+double cc_es_norm(double alpha, double betaw) { return es_norm(alpha, betaw); }
+double cc_dipole_thresh(double alpha, double betaw) { return dipole_thresh(alpha, betaw); }
+
+// z-extended grid tables; arrays sized 3*Nz
+int cc_zext(const double* zG, const double* zW, int Nz, double thresh, int wall_top, int wall_bot,
+            double* z, double* w, int* kt, double* facA, double* facB, int* info /*up,down,wzmax*/)
+{
+  std::vector<double> g(zG, zG + Nz), ww(zW, zW + Nz);
+  ZExt X = build_zext(g, ww, thresh, wall_top != 0, wall_bot != 0);
+  for (int e = 0; e < X.nzeff; ++e) {
+    z[e] = X.z[e]; w[e] = X.w[e]; kt[e] = X.kt[e]; facA[e] = X.facA[e]; facB[e] = X.facB[e];
+  }
+  info[0] = X.ext_up; info[1] = X.ext_down; info[2] = X.wzmax;
+  return X.nzeff;
+}
+
+// per-particle supports and weights for N particles (positions already in box)
+void cc_stencil(int N, const double* pos, double hx, double hy, double betaw, double alpha, double thresh,
+                double inv_norm3, int wx, int wy, const double* zext, const double* zwext, int nzeff,
+                int wzmax, int* sup /*4N*/, double* wxo, double* dxo, double* wyo, double* dyo,
+                double* wzo, double* dzo, double* wzio, double* dzio)
+{
+  KernelDesc K;
+  K.betaw = betaw; K.alpha = alpha; K.thresh = thresh; K.inv_norm3 = inv_norm3;
+  K.wx = wx; K.wy = wy; K.nzeff = nzeff; K.ext_up = 0; K.wzmax = wzmax;
+  for (int p = 0; p < N; ++p) {
+    const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    Stencil s = stencil_support(x, y, z, hx, hy, K, zext);
+    sup[4 * p] = s.bx; sup[4 * p + 1] = s.by; sup[4 * p + 2] = s.e0; sup[4 * p + 3] = s.nz;
+    stencil_weights_xy(x, hx, s.bx, wx, K, inv_norm3, wxo + p * wx, dxo + p * wx);
+    stencil_weights_xy(y, hy, s.by, wy, K, 1.0, wyo + p * wy, dyo + p * wy);
+    stencil_weights_z(z, s, K, zext, zwext, hx * hy, wzo + p * wzmax, dzo + p * wzmax,
+                      wzio + p * wzmax, dzio + p * wzmax);
+  }
+}
+
+// BVP for M modes given their wavenumbers.  fin/uout: [M][3][Nz][2];
+// corr: [M][3][4][2].  Mode with kx = ky = 0 runs the k = 0 branch.
+void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const double* kx,
+            const double* ky, const int* nyqx, const int* nyqy, const double* fin, double* uout,
+            double* corr)
+{
+  ChebTables T = build_cheb_tables(Nz, H);
+  BvpConsts B;
+  B.Nz = Nz; B.geometry = geometry; B.H = H; B.Hs = Hs; B.eta = eta;
+  B.sd0 = T.sd0.data(); B.sm2 = T.sm2.data(); B.sp2 = T.sp2.data();
+  B.S0N = T.S0N; B.S1N1 = T.S1N1;
+  B.fm1 = T.fm1.data(); B.fp1 = T.fp1.data(); B.F0N1 = T.F0N1;
+  B.R1 = T.R1.data(); B.R2 = T.R2.data(); B.L1 = T.L1.data(); B.L2 = T.L2.data();
+  std::vector<double> beta(Nz + 1), ab(Nz);
+  std::vector<cplx> P(Nz), CP(Nz), DP(Nz), U(Nz);
+  ScratchView sc;
+  sc.beta = beta.data(); sc.ab = ab.data(); sc.P = P.data(); sc.CP = CP.data(); sc.DP = DP.data();
+  sc.U = U.data(); sc.stride = 1;
+  for (int m = 0; m < M; ++m) {
+    HostIn in{fin + (size_t)m * 6 * Nz, Nz, false};
+    HostOut out{uout + (size_t)m * 6 * Nz, Nz};
+    ModeResult r;
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
+    if (kx[m] == 0 && ky[m] == 0) bvp_k0(B, in, out, U.data(), 1);
+    else bvp_mode(B, kx[m], ky[m], nyqx[m], nyqy[m], in, out, sc, r);
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 4; ++b) {
+        corr[(size_t)m * 24 + (c * 4 + b) * 2] = r.corr[c][b].re;
+        corr[(size_t)m * 24 + (c * 4 + b) * 2 + 1] = r.corr[c][b].im;
+      }
+  }
+}
+}
