@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark: doubly periodic Stokes Mdot on one MI355X.
+
+Workload (default): slit channel, walls at z=0 and z=32, N=16384 particles,
+Lx=Ly=128, x,y uniform, z uniform in [1.5, 30.5], grid 256x256x96, ES kernels
+w=w_d=6 with beta*w=10.71 for forces and torques, forces and torques iid
+N(0,1) (numpy RandomState(0), draw order x, y, z, F, T).  A step is one Mdot
+(forces + torques -> velocities + angular velocities) with inputs resident
+in HBM, mixed precision (FP32 grids/FFT/z-transform, FP64 BVP).
+
+Prints one JSON line.  Besides throughput it reports the relative L2
+difference of the mixed path against the FP64 GPU path on the same inputs,
+per-stage times from HIP events on the solver's stream, the roofline line of
+the dominant kernel, and an FP64 CPU baseline (the oracle restatement of the
+reference CPU algorithm) timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (geometry, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque)
+    "slit16k": ("slit", 16384, 128.0, 128.0, 32.0, 1.5, 30.5, 256, 256, 96, True),
+    "bottom2k": ("bottom", 2048, 64.0, 64.0, 24.0, 1.5, 20.0, 128, 128, 64, False),
+}
+
+
+def make_inputs(wl, seed=0):
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    rng = np.random.RandomState(seed)
+    x = rng.uniform(0, Lx, N)
+    y = rng.uniform(0, Ly, N)
+    z = rng.uniform(zlo, zhi, N)
+    F = rng.normal(0, 1, 3 * N)
+    T = rng.normal(0, 1, 3 * N) if torque else None
+    return np.stack([x, y, z], 1).ravel(), F, T
+
+
+def stage_bytes(wl, s):
+    """Algorithmic HBM bytes of each stage (s = bytes per grid real)."""
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    ng = 3 * nx * ny * nz
+    nm = 3 * nz * ny * (nx // 2 + 1)
+    npart = N * (6 if torque else 3)
+    return {
+        "spread": ng * s + npart * s,
+        "fft_fwd": ng * s + nm * 2 * s,
+        "z_fwd": 2 * nm * 2 * s,
+        "bvp": 2 * nm * 2 * s,
+        "z_inv": 2 * nm * 2 * s,
+        "fft_inv": nm * 2 * s + ng * s,
+        "interp": ng * s + npart * s,
+    }
+
+
+def gpu_solver(wl, precision):
+    from doublyperiodicstokes_amd import DPStokes, StokesParameters
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    par = StokesParameters(viscosity=1 / (6 * np.pi), Lx=Lx, Ly=Ly, zmin=0.0, zmax=zmax,
+                           w=6, w_d=6 if torque else 0, beta=1.785 * 6, beta_d=1.785 * 6,
+                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision)
+    s = DPStokes()
+    s.initialize(par, N)
+    return s
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1000.0 / steps
+
+
+def cpu_baseline(wl, pos, F, T, budget_s):
+    """FP64 oracle Mdot timed on the host: as many full calls as fit in the budget (at least one)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import dp_oracle as o
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    np.seterr(all="ignore")
+    ref = o.OracleDPStokes(geom, Lx, Ly, 0.0, zmax, nx, ny, nz, 1 / (6 * np.pi), 6, 1.785 * 6,
+                           has_torque=torque, w_d=6 if torque else None,
+                           betaw_d=1.785 * 6 if torque else None)
+    ref.set_positions(pos)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        ref.mdot(F, T)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 20:
+            break
+    return n / el, n, el, o.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="slit16k", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU-baseline Mdot work (0 disables)")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    wl = WORKLOADS[args.workload]
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    pos, F, T = make_inputs(wl)
+    dev = torch.device("cuda", 0)
+    Fd = torch.from_numpy(F.astype(np.float32)).to(dev)
+    Td = torch.from_numpy(T.astype(np.float32)).to(dev) if torque else None
+
+    mixed = gpu_solver(wl, "mixed")
+    mixed.setPositions(pos)
+    ms = time_steps(torch, lambda: mixed.mdot_device(Fd, Td), args.steps, args.warmup)
+    U32, W32 = mixed.mdot_device(Fd, Td)
+    torch.cuda.synchronize()
+
+    # per-stage times from HIP events recorded on the solver's stream
+    mixed.set_profiling(True)
+    acc = None
+    nprof = 10
+    for _ in range(nprof):
+        mixed.mdot_device(Fd, Td)
+        t = mixed.timings()
+        acc = t if acc is None else {k: acc[k] + t[k] for k in acc}
+    mixed.set_profiling(False)
+    stage_us = {k: v / nprof for k, v in acc.items() if k != "locate"}
+    sb = stage_bytes(wl, 4)
+    stage_gbs = {k: sb[k] / (stage_us[k] * 1e-6) / 1e9 for k in stage_us}
+    dom = max(stage_us, key=lambda k: stage_us[k])
+
+    out = {
+        "metric": "Mdot solves/sec, N=16k slit, mixed vs FP64 at rel-L2<=1e-5; spread HBM GB/s",
+        "value": 1000.0 / ms,
+        "unit": "Mdot/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grids + f64 BVP",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s"
+                   % (args.workload, geom, N, Lx, Ly, zmax, nx, ny, nz, "F+T" if torque else "F"),
+                   "precision": "mixed"},
+        "stage_us": stage_us,
+        "stage_GBps": stage_gbs,
+        "spread_GBps": stage_gbs["spread"],
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": stage_gbs[dom],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stage_gbs[dom] / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": sb[dom]},
+    }
+    del mixed
+    if not args.no_fp64:
+        f64 = gpu_solver(wl, "double")
+        f64.setPositions(pos)
+        Fd64 = torch.from_numpy(F).to(dev)
+        Td64 = torch.from_numpy(T).to(dev) if torque else None
+        ms64 = time_steps(torch, lambda: f64.mdot_device(Fd64, Td64), max(5, args.steps // 5), 3)
+        U64, W64 = f64.mdot_device(Fd64, Td64)
+        torch.cuda.synchronize()
+        u64 = U64.cpu().numpy()
+        out["fp64_ms_per_step"] = ms64
+        out["rel_l2_U_mixed_vs_fp64"] = float(np.linalg.norm(U32.double().cpu().numpy() - u64) / np.linalg.norm(u64))
+        if torque:
+            w64 = W64.cpu().numpy()
+            out["rel_l2_W_mixed_vs_fp64"] = float(np.linalg.norm(W32.double().cpu().numpy() - w64) / np.linalg.norm(w64))
+        del f64
+    if args.cpu_budget > 0:
+        rate, n, el, thr = cpu_baseline(wl, pos, F, T, args.cpu_budget)
+        out["cpu_baseline"] = {"value": rate, "unit": "Mdot/s", "cores": thr, "kind": "port",
+                               "sample": "%d full-size FP64 oracle Mdot call(s) on the same inputs (%.1f s)"
+                               % (n, el)}
+        out["speedup_vs_cpu"] = out["value"] / rate
+        if "fp64_ms_per_step" in out:
+            out["fp64_speedup_vs_cpu"] = (1000.0 / out["fp64_ms_per_step"]) / rate
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

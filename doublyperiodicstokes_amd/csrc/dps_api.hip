@@ -71,6 +71,7 @@ struct dps_handle {
   int4 *d_sup_m = nullptr, *d_sup_d = nullptr;
   void *d_rec_m = nullptr, *d_rec_d = nullptr;
   int rs_m = 0, rs_d = 0;
+  int4* d_zrange = nullptr;
   int *d_key = nullptr, *d_rank = nullptr, *d_counts = nullptr, *d_start = nullptr, *d_order = nullptr;
   void* d_grid = nullptr;
   void *d_specA = nullptr, *d_specB = nullptr;
@@ -81,12 +82,12 @@ struct dps_handle {
   double *d_zpts = nullptr, *d_kxs = nullptr, *d_kys = nullptr;
   double *d_sd0 = nullptr, *d_sm2 = nullptr, *d_sp2 = nullptr, *d_fm1 = nullptr, *d_fp1 = nullptr;
   double *d_R1 = nullptr, *d_R2 = nullptr, *d_L1 = nullptr, *d_L2 = nullptr;
-  int *d_is_special = nullptr, *d_special = nullptr;
+  int* d_special = nullptr;
   int nspecial = 0;
   double2* d_u2 = nullptr;
-  double *s_beta = nullptr, *s_ab = nullptr;
-  double2 *s_P = nullptr, *s_CP = nullptr, *s_DP = nullptr, *s_U = nullptr;
-  double2* d_corr = nullptr;
+  double *s_ib = nullptr, *s_ab = nullptr;
+  double2 *s_P = nullptr, *s_CP = nullptr, *s_DP = nullptr, *s_U0 = nullptr, *s_U1 = nullptr, *s_U2 = nullptr;
+  double2 *d_corr = nullptr, *d_corr2 = nullptr;
   double* d_stage = nullptr;   // FP64 reference-layout staging for stage entry points
 
   rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
@@ -249,6 +250,7 @@ int allocate(dps_handle* h)
     h->rs_d = 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
     if ((rc = h->alloc((unsigned char**)&h->d_rec_d, (size_t)N * h->rs_d * ts))) return rc;
   }
+  if ((rc = h->alloc(&h->d_zrange, N))) return rc;
   if ((rc = h->alloc(&h->d_key, N))) return rc;
   if ((rc = h->alloc(&h->d_rank, N))) return rc;
   if ((rc = h->alloc(&h->d_order, N))) return rc;
@@ -309,29 +311,28 @@ int allocate(dps_handle* h)
   if ((rc = h->upload(&h->d_R2, h->cheb.R2))) return rc;
   if ((rc = h->upload(&h->d_L1, h->cheb.L1))) return rc;
   if ((rc = h->upload(&h->d_L2, h->cheb.L2))) return rc;
-  // modes that touch a Nyquist index
-  std::vector<int> is_sp(h->Nm, 0), sp;
+  // modes that touch a Nyquist index (their mirror is solved separately)
+  std::vector<int> sp;
   for (int iy = 0; iy < h->Ny; ++iy)
     for (int ix = 0; ix < h->Nkx; ++ix) {
       const bool s = ((h->Nx % 2 == 0) && ix == h->Nx / 2) || ((h->Ny % 2 == 0) && iy == h->Ny / 2);
-      if (s && !(ix == 0 && iy == 0)) {
-        is_sp[iy * h->Nkx + ix] = 1;
-        sp.push_back(iy * h->Nkx + ix);
-      }
+      if (s && !(ix == 0 && iy == 0)) sp.push_back(iy * h->Nkx + ix);
     }
   h->nspecial = (int)sp.size();
-  if ((rc = h->upload(&h->d_is_special, is_sp))) return rc;
   if ((rc = h->upload(&h->d_special, sp))) return rc;
   if ((rc = h->alloc(&h->d_u2, (size_t)std::max(1, h->nspecial) * 3 * h->Nz))) return rc;
-  // BVP scratch and correction coefficients
-  const size_t nm = (size_t)h->Nm;
-  if ((rc = h->alloc(&h->s_beta, (size_t)(h->Nz + 1) * nm))) return rc;
-  if ((rc = h->alloc(&h->s_ab, (size_t)h->Nz * nm))) return rc;
-  if ((rc = h->alloc(&h->s_P, (size_t)h->Nz * nm))) return rc;
-  if ((rc = h->alloc(&h->s_CP, (size_t)h->Nz * nm))) return rc;
-  if ((rc = h->alloc(&h->s_DP, (size_t)h->Nz * nm))) return rc;
-  if ((rc = h->alloc(&h->s_U, (size_t)h->Nz * nm))) return rc;
-  if ((rc = h->alloc(&h->d_corr, nm * 12))) return rc;
+  if ((rc = h->alloc(&h->d_corr2, (size_t)std::max(1, h->nspecial) * 12))) return rc;
+  // BVP scratch (one slot per lane) and correction coefficients
+  const size_t ms = (size_t)h->Nm + h->nspecial;
+  if ((rc = h->alloc(&h->s_ib, (size_t)(h->Nz + 1) * ms))) return rc;
+  if ((rc = h->alloc(&h->s_ab, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_P, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_CP, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_DP, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_U0, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_U1, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->s_U2, (size_t)h->Nz * ms))) return rc;
+  if ((rc = h->alloc(&h->d_corr, (size_t)h->Nm * 12))) return rc;
   // rocFFT plans: 2-D over (x, y), batched over (component, z)
   const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
   size_t lengths[2] = {(size_t)h->Nx, (size_t)h->Ny};
@@ -394,6 +395,12 @@ StencilArgs stencil_args(dps_handle* h)
   A.TX = h->TX;
   A.TY = h->TY;
   A.ntx = h->ntx;
+  A.kt_m = h->d_kt_m;
+  A.kt_d = h->d_kt_d;
+  A.fA_m = h->d_fA_m;
+  A.fA_d = h->d_fA_d;
+  A.fB_d = h->d_fB_d;
+  A.zrange = h->d_zrange;
   return A;
 }
 
@@ -403,6 +410,7 @@ SpreadArgs spread_args(dps_handle* h)
   A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
   A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty; A.RX = h->RX; A.RY = h->RY;
   A.start = h->d_start; A.order = h->d_order;
+  A.zrange = h->d_zrange;
   A.km = h->km; A.kd = h->kd;
   A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
   A.rec_m = h->d_rec_m; A.rec_d = h->d_rec_d; A.rs_m = h->rs_m; A.rs_d = h->rs_d;
@@ -440,7 +448,7 @@ ZArgs z_args(dps_handle* h)
   return A;
 }
 
-BvpArgs bvp_args(dps_handle* h, bool special)
+BvpArgs bvp_args(dps_handle* h)
 {
   BvpArgs A;
   BvpConsts& B = A.B;
@@ -451,13 +459,15 @@ BvpArgs bvp_args(dps_handle* h, bool special)
   B.R1 = h->d_R1; B.R2 = h->d_R2; B.L1 = h->d_L1; B.L2 = h->d_L2;
   A.Nx = h->Nx; A.Ny = h->Ny; A.Nkx = h->Nkx; A.Nm = h->Nm;
   A.Lx = h->P.Lx; A.Ly = h->P.Ly;
-  A.is_special = h->d_is_special;
-  A.special = special ? h->d_special : nullptr;
+  A.Ms = h->Nm + h->nspecial;
+  A.special = h->d_special;
   A.nspecial = h->nspecial;
   A.u2 = h->d_u2;
-  A.s_beta = h->s_beta; A.s_ab = h->s_ab;
-  A.s_P = h->s_P; A.s_CP = h->s_CP; A.s_DP = h->s_DP; A.s_U = h->s_U;
+  A.s_ib = h->s_ib; A.s_ab = h->s_ab;
+  A.s_P = h->s_P; A.s_CP = h->s_CP; A.s_DP = h->s_DP;
+  A.s_U0 = h->s_U0; A.s_U1 = h->s_U1; A.s_U2 = h->s_U2;
   A.corr = h->d_corr;
+  A.corr2 = h->d_corr2;
   return A;
 }
 
@@ -491,8 +501,8 @@ int do_solve(dps_handle* h, hipStream_t s, int ev0)
   else launch_zfwd<double>(Z, h->d_specA, h->d_specB, s);
   HIPCHK(hipGetLastError());
   mark(h, ev0 + 2, s);
-  for (int sp = 0; sp < 2; ++sp) {
-    const BvpArgs B = bvp_args(h, sp == 1);
+  {
+    const BvpArgs B = bvp_args(h);
     if (mixed(h)) launch_bvp<float>(B, h->d_specB, h->d_specA, s);
     else launch_bvp<double>(B, h->d_specB, h->d_specA, s);
     HIPCHK(hipGetLastError());

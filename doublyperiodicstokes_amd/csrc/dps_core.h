@@ -183,16 +183,19 @@ struct BvpConsts {
 };
 
 // Mode-local arrays are accessed through this view: element i of array `a`
-// lives at base[a][i * stride] (stride = number of modes, so that
-// neighbouring modes are contiguous).  On the host the same view runs with
-// stride 1 over private buffers.
+// lives at a[i * stride] (stride = number of modes, so that neighbouring modes
+// are contiguous and one wavefront's accesses coalesce).  On the host the same
+// view runs with stride 1 over private buffers.  Every recurrence carries its
+// last two values in registers, so loads never wait on the store before them.
 struct ScratchView {
-  double* beta;   // Nz+1
-  double* ab;     // Nz  (A^{-1}B, column 0 on even rows, column 1 on odd rows)
-  cplx* P;        // Nz  pressure rhs / solution
-  cplx* CP;       // Nz  pressure coefficients
-  cplx* DP;       // Nz  z-derivative of pressure coefficients
-  cplx* U;        // Nz  velocity work vector
+  double* ib;     // Nz+1  reciprocal KBPENTA pivots 1/beta[i]
+  double* ab;     // Nz    A^{-1}B (column 0 on even rows, column 1 on odd rows)
+  cplx* P;        // Nz    pressure rhs / sweep / solution
+  cplx* CP;       // Nz    pressure coefficients
+  cplx* DP;       // Nz    z-derivative of the pressure coefficients
+  cplx* U0;       // Nz    velocity sweeps, one per component
+  cplx* U1;
+  cplx* U2;
   long stride;
 };
 
@@ -207,6 +210,12 @@ struct ModeResult {
   int nbasis;
 };
 
+// Each pass over z is processed in chunks of CH steps: all loads of a chunk
+// are issued first, then the dependent arithmetic runs on registers, then the
+// stores -- so a lane keeps CH memory requests in flight instead of waiting
+// one latency per recurrence step.
+#define DPS_CH 8
+
 template <class In, class Out>
 DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int nyq_y,
                      const In& fin_, Out& fout, const ScratchView& sc, ModeResult& res)
@@ -218,112 +227,206 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
   const double K = sqrt(Ksq);
   const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
   const cplx Dy = nyq_y ? mk(0, 0) : mk(0, ky);
+  constexpr int CH = DPS_CH;
 #define D0(i) (1.0 - B.sd0[(i)] * Ksq)
 #define DP2(i) (0.0 - B.sp2[(i)] * Ksq)
 #define DM2(i) (0.0 - B.sm2[(i) + 2] * Ksq)
-  // KBPENTA pivots (LinearSolvers.h:52-74)
-  double* beta = sc.beta;
-  beta[0] = 0;
-  beta[1 * st] = D0(0);
-  beta[2 * st] = D0(1);
+#define IB(i) sc.ib[(long)(i) * st]
+#define AB(i) sc.ab[(long)(i) * st]
+#define PP(i) sc.P[(long)(i) * st]
+  // KBPENTA pivots (LinearSolvers.h:52-74), stored as reciprocals
   {
-    double bm2 = D0(0), bm1 = D0(1);   // beta[i-2], beta[i-1]
+    double bm2 = D0(0), bm1 = D0(1);
+    IB(1) = 1.0 / bm2;
+    IB(2) = 1.0 / bm1;
     for (int i = 3; i <= Nz; ++i) {
       const double b = D0(i - 1) - DM2(i - 3) * DP2(i - 3) / bm2;
-      beta[(long)i * st] = b;
+      IB(i) = 1.0 / b;
       bm2 = bm1;
       bm1 = b;
-    }
-  }
-  // A^{-1}B (LinearSolvers.cpp:42): B = -Ksq * SIMat[:, Nz:Nz+2] has one
-  // non-zero per column (rows 0 and 1), so the two solutions live on even and
-  // odd rows and are stored merged.
-  double* ab = sc.ab;
-  {
-    double x0 = B.S0N * (-Ksq), x1 = B.S1N1 * (-Ksq);
-    ab[0] = x0;
-    ab[st] = x1;
-    double xm2 = x0, xm1 = x1;
-    for (int i = 2; i < Nz; ++i) {
-      const double v = 0.0 - DM2(i - 2) * xm2 / beta[(long)(i - 1) * st];
-      ab[(long)i * st] = v;
-      xm2 = xm1;
-      xm1 = v;
-    }
-    ab[(long)(Nz - 1) * st] /= beta[(long)Nz * st];
-    ab[(long)(Nz - 2) * st] /= beta[(long)(Nz - 1) * st];
-    double xp2 = ab[(long)(Nz - 1) * st], xp1 = ab[(long)(Nz - 2) * st];
-    for (int i = Nz - 3; i >= 0; --i) {
-      const double v = (ab[(long)i * st] - DP2(i) * xp2) / beta[(long)(i + 1) * st];
-      ab[(long)i * st] = v;
-      xp2 = xp1;
-      xp1 = v;
     }
   }
   // Robin BC rows H^2 K BCR2 + H BCR1 and H^2 K BCL2 + H BCL1 (Solvers.py:309-313)
   const double H2K = (H * H) * K;
 #define C0(j) (B.R2[(j)] * H2K + (B.R2[(j)] + (H * B.R1[(j)] - B.R2[(j)])))
 #define C1(j) (B.L2[(j)] * H2K + (-B.L2[(j)] + (H * B.L1[(j)] + B.L2[(j)])))
+  // A^{-1}B (LinearSolvers.cpp:42): B = -Ksq SIMat[:, Nz:Nz+2] has one non-zero
+  // per column (rows 0 and 1), so the two solutions live on even and odd rows.
+  // G = C A^{-1}B - D and its explicit inverse (LinearSolvers.cpp:44-56).
   double G00 = 0, G01 = 0, G10 = 0, G11 = 0;
-  for (int j = 0; j < Nz; ++j) {
-    const double a = ab[(long)j * st];
-    if (j & 1) { G01 += C0(j) * a; G11 += C1(j) * a; }
-    else { G00 += C0(j) * a; G10 += C1(j) * a; }
+  {
+    double xm2 = B.S0N * (-Ksq), xm1 = B.S1N1 * (-Ksq);
+    AB(0) = xm2;
+    AB(1) = xm1;
+    for (int i0 = 2; i0 < Nz; i0 += CH) {
+      double ibv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 + u < Nz) ibv[u] = IB(i0 + u - 1);
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + u;
+        if (i < Nz) {
+          const double v = 0.0 - (DM2(i - 2) * ibv[u]) * xm2;
+          AB(i) = v;
+          xm2 = xm1;
+          xm1 = v;
+        }
+      }
+    }
+    double xp2 = xm1 * IB(Nz), xp1 = xm2 * IB(Nz - 1);   // x[Nz-1], x[Nz-2]
+    AB(Nz - 1) = xp2;
+    AB(Nz - 2) = xp1;
+    if ((Nz - 1) & 1) { G01 += C0(Nz - 1) * xp2; G11 += C1(Nz - 1) * xp2; G00 += C0(Nz - 2) * xp1; G10 += C1(Nz - 2) * xp1; }
+    else { G00 += C0(Nz - 1) * xp2; G10 += C1(Nz - 1) * xp2; G01 += C0(Nz - 2) * xp1; G11 += C1(Nz - 2) * xp1; }
+    for (int i0 = Nz - 3; i0 >= 0; i0 -= CH) {
+      double abv[CH], ibv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 - u >= 0) { abv[u] = AB(i0 - u); ibv[u] = IB(i0 - u + 1); }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 - u;
+        if (i >= 0) {
+          const double v = (abv[u] - DP2(i) * xp2) * ibv[u];
+          abv[u] = v;
+          if (i & 1) { G01 += C0(i) * v; G11 += C1(i) * v; }
+          else { G00 += C0(i) * v; G10 += C1(i) * v; }
+          xp2 = xp1;
+          xp1 = v;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 - u >= 0) AB(i0 - u) = abv[u];
+    }
   }
   G00 -= C0(Nz); G10 -= C1(Nz); G01 -= C0(Nz + 1); G11 -= C1(Nz + 1);
   const double det = 1.0 / (G00 * G11 - G01 * G10);
   const double Gi00 = G11 * det, Gi10 = -G10 * det, Gi01 = -G01 * det, Gi11 = G00 * det;
 
   // ---- pressure: chebCoeffDiff + assemble_p_rhs (DPTools.cpp:134-195) ----
-  cplx* P = sc.P;
   {
     cplx dfp1 = mk(0, 0), dfp2 = mk(0, 0);   // df[j+1], df[j+2]
-    for (int j = Nz - 1; j >= 0; --j) {
-      cplx df;
-      if (j == Nz - 1) df = mk(0, 0);
-      else if (j == Nz - 2) df = ((double)((Nz - 1) * 2) / H) * fin(fin_, 2, Nz - 1);
-      else df = dfp2 + ((double)((j + 1) * 2) / H) * fin(fin_, 2, j + 1);
-      dfp2 = dfp1;
-      dfp1 = df;
-      if (j == 0) df = df / 2;
-      P[(long)j * st] = Dx * fin(fin_, 0, j) + Dy * fin(fin_, 1, j) + df;
+    cplx fz_next = mk(0, 0);                 // f_z[j+1]
+    for (int j0 = Nz - 1; j0 >= 0; j0 -= CH) {
+      cplx fx[CH], fy[CH], fz[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (j0 - u >= 0) { fx[u] = fin(fin_, 0, j0 - u); fy[u] = fin(fin_, 1, j0 - u); fz[u] = fin(fin_, 2, j0 - u); }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int j = j0 - u;
+        if (j >= 0) {
+          cplx df;
+          if (j == Nz - 1) df = mk(0, 0);
+          else if (j == Nz - 2) df = ((double)((Nz - 1) * 2) / H) * fz_next;
+          else df = dfp2 + ((double)((j + 1) * 2) / H) * fz_next;
+          dfp2 = dfp1;
+          dfp1 = df;
+          fz_next = fz[u];
+          if (j == 0) df = df / 2;
+          fx[u] = Dx * fx[u] + Dy * fy[u] + df;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (j0 - u >= 0) PP(j0 - u) = fx[u];
     }
   }
-  // KBPENTA forward / backward sweeps on P (LinearSolvers.h:106-141)
-  for (int i = 2; i < Nz; ++i)
-    P[(long)i * st] = P[(long)i * st] - (DM2(i - 2) * P[(long)(i - 2) * st]) / beta[(long)(i - 1) * st];
-  P[(long)(Nz - 1) * st] = P[(long)(Nz - 1) * st] / beta[(long)Nz * st];
-  P[(long)(Nz - 2) * st] = P[(long)(Nz - 2) * st] / beta[(long)(Nz - 1) * st];
-  for (int i = Nz - 3; i >= 0; --i)
-    P[(long)i * st] = (P[(long)i * st] - DP2(i) * P[(long)(i + 2) * st]) / beta[(long)(i + 1) * st];
+  // KBPENTA forward / backward sweeps (LinearSolvers.h:106-141)
+  {
+    cplx gm2 = PP(0), gm1 = PP(1);
+    for (int i0 = 2; i0 < Nz; i0 += CH) {
+      cplx v[CH];
+      double ibv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 + u < Nz) { v[u] = PP(i0 + u); ibv[u] = IB(i0 + u - 1); }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + u;
+        if (i < Nz) {
+          const cplx g = v[u] - (DM2(i - 2) * ibv[u]) * gm2;
+          v[u] = g;
+          gm2 = gm1;
+          gm1 = g;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 + u < Nz) PP(i0 + u) = v[u];
+    }
+  }
   cplx y0 = mk(0, 0), y1 = mk(0, 0);
-  for (int j = 0; j < Nz; ++j) {
-    const cplx x = P[(long)j * st];
-    y0 = y0 + x * C0(j);
-    y1 = y1 + x * C1(j);
+  {
+    cplx xp2 = PP(Nz - 1) * IB(Nz), xp1 = PP(Nz - 2) * IB(Nz - 1);
+    PP(Nz - 1) = xp2;
+    PP(Nz - 2) = xp1;
+    y0 = xp2 * C0(Nz - 1) + xp1 * C0(Nz - 2);
+    y1 = xp2 * C1(Nz - 1) + xp1 * C1(Nz - 2);
+    for (int i0 = Nz - 3; i0 >= 0; i0 -= CH) {
+      cplx v[CH];
+      double ibv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 - u >= 0) { v[u] = PP(i0 - u); ibv[u] = IB(i0 - u + 1); }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 - u;
+        if (i >= 0) {
+          const cplx x = (v[u] - DP2(i) * xp2) * ibv[u];
+          v[u] = x;
+          y0 = y0 + x * C0(i);
+          y1 = y1 + x * C1(i);
+          xp2 = xp1;
+          xp1 = x;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (i0 - u >= 0) PP(i0 - u) = v[u];
+    }
   }
   const cplx Y0 = Gi00 * y0 + Gi01 * y1;
   const cplx Y1 = Gi10 * y0 + Gi11 * y1;
-  // x = xp - A^{-1}B Y ; Cp = SIMat x ; Dp = FIMat x  (DPTools.cpp:266-284)
-  cplx* CP = sc.CP;
-  cplx* DPv = sc.DP;
+  // x = xp - A^{-1}B Y ; Cp = SIMat x ; Dp = FIMat x  (DPTools.cpp:266-284),
+  // streamed with a five-point window x[m-2..m+2]
   cplx psum = mk(0, 0), palt = mk(0, 0);
   {
-#define XP(j) ((j) < Nz ? (P[(long)(j) * st] - ((((j) & 1) ? ab[(long)(j) * st] * Y1 : ab[(long)(j) * st] * Y0))) \
-                        : ((j) == Nz ? Y0 : Y1))
-    for (int m = 0; m < Nz; ++m) {
-      cplx cp = B.sd0[m] * XP(m);
-      if (m >= 2) cp = B.sm2[m] * XP(m - 2) + cp;
-      cp = cp + B.sp2[m] * XP(m + 2);
-      if (m == 0) cp = cp + B.S0N * Y0;
-      if (m == 1) cp = cp + B.S1N1 * Y1;
-      cplx dp = B.fp1[m] * XP(m + 1);
-      if (m >= 1) dp = B.fm1[m] * XP(m - 1) + dp;
-      if (m == 0) dp = dp + B.F0N1 * Y1;
-      CP[(long)m * st] = cp;
-      DPv[(long)m * st] = dp;
-      psum = psum + cp;
-      palt = (m & 1) ? palt - cp : palt + cp;
+#define XP(j) ((j) < Nz ? (PP(j) - ((((j) & 1) ? AB(j) * Y1 : AB(j) * Y0))) : ((j) == Nz ? Y0 : Y1))
+    cplx xm2 = mk(0, 0), xm1 = mk(0, 0), x0 = XP(0), x1 = XP(1), x2 = XP(2);
+    for (int m0 = 0; m0 < Nz; m0 += CH) {
+      cplx ahead[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int j = m0 + u + 3;
+        if (m0 + u < Nz && j <= Nz + 1) ahead[u] = XP(j);
+      }
+      cplx cpv[CH], dpv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int m = m0 + u;
+        if (m < Nz) {
+          cplx cp = B.sd0[m] * x0;
+          if (m >= 2) cp = B.sm2[m] * xm2 + cp;
+          cp = cp + B.sp2[m] * x2;
+          if (m == 0) cp = cp + B.S0N * Y0;
+          if (m == 1) cp = cp + B.S1N1 * Y1;
+          cplx dp = B.fp1[m] * x1;
+          if (m >= 1) dp = B.fm1[m] * xm1 + dp;
+          if (m == 0) dp = dp + B.F0N1 * Y1;
+          cpv[u] = cp;
+          dpv[u] = dp;
+          psum = psum + cp;
+          palt = (m & 1) ? palt - cp : palt + cp;
+          xm2 = xm1; xm1 = x0; x0 = x1; x1 = x2;
+          if (m + 3 <= Nz + 1) x2 = ahead[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (m0 + u < Nz) { sc.CP[(long)(m0 + u) * st] = cpv[u]; sc.DP[(long)(m0 + u) * st] = dpv[u]; }
     }
 #undef XP
   }
@@ -338,44 +441,147 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
   bc[4] = psum / fac1;
   bc[5] = palt / fac1;
 
-  // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345) ----
+  // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345),
+  //      the three components interleaved ----
+  cplx* Uc[3] = {sc.U0, sc.U1, sc.U2};
+  {
+    constexpr int CV = DPS_CH / 2;
+    cplx gm2[3], gm1[3];
+    for (int j0 = 0; j0 < Nz; j0 += CV) {
+      cplx cpv[CV], dpv[CV], f0[CV], f1[CV], f2[CV];
+      double ibv[CV];
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int j = j0 + u;
+        if (j < Nz) {
+          cpv[u] = sc.CP[(long)j * st];
+          dpv[u] = sc.DP[(long)j * st];
+          f0[u] = fin(fin_, 0, j);
+          f1[u] = fin(fin_, 1, j);
+          f2[u] = fin(fin_, 2, j);
+          ibv[u] = (j >= 2) ? IB(j - 1) : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int j = j0 + u;
+        if (j < Nz) {
+          cplx r[3];
+          r[0] = (Dx * cpv[u] - f0[u]) / B.eta;
+          r[1] = (Dy * cpv[u] - f1[u]) / B.eta;
+          r[2] = (dpv[u] - f2[u]) / B.eta;
+          const double l = (j >= 2) ? DM2(j - 2) * ibv[u] : 0.0;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            cplx g = r[c];
+            if (j >= 2) g = g - l * gm2[c];
+            gm2[c] = (j >= 1) ? gm1[c] : mk(0, 0);
+            gm1[c] = g;
+          }
+          f0[u] = gm1[0];
+          f1[u] = gm1[1];
+          f2[u] = gm1[2];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int j = j0 + u;
+        if (j < Nz) {
+          Uc[0][(long)j * st] = f0[u];
+          Uc[1][(long)j * st] = f1[u];
+          Uc[2][(long)j * st] = f2[u];
+        }
+      }
+    }
+  }
+  cplx Ya[3], Yb[3];
+  {
+    constexpr int CV = DPS_CH / 2;
+    cplx xp2[3], xp1[3], ya[3], yb[3];
+    const double i1 = IB(Nz), i2 = IB(Nz - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xp2[c] = Uc[c][(long)(Nz - 1) * st] * i1;
+      xp1[c] = Uc[c][(long)(Nz - 2) * st] * i2;
+      Uc[c][(long)(Nz - 1) * st] = xp2[c];
+      Uc[c][(long)(Nz - 2) * st] = xp1[c];
+      ya[c] = xp2[c] * C0(Nz - 1) + xp1[c] * C0(Nz - 2);
+      yb[c] = xp2[c] * C1(Nz - 1) + xp1[c] * C1(Nz - 2);
+    }
+    for (int i0 = Nz - 3; i0 >= 0; i0 -= CV) {
+      cplx v[3][CV];
+      double ibv[CV];
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int i = i0 - u;
+        if (i >= 0) {
+          ibv[u] = IB(i + 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) v[c][u] = Uc[c][(long)i * st];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int i = i0 - u;
+        if (i >= 0) {
+          const double d = DP2(i), c0 = C0(i), c1 = C1(i);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const cplx x = (v[c][u] - d * xp2[c]) * ibv[u];
+            v[c][u] = x;
+            ya[c] = ya[c] + x * c0;
+            yb[c] = yb[c] + x * c1;
+            xp2[c] = xp1[c];
+            xp1[c] = x;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CV; ++u) {
+        const int i = i0 - u;
+        if (i >= 0)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Uc[c][(long)i * st] = v[c][u];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cplx a = ya[c] - bc[2 * c], b2 = yb[c] - bc[2 * c + 1];
+      Ya[c] = Gi00 * a + Gi01 * b2;
+      Yb[c] = Gi10 * a + Gi11 * b2;
+    }
+  }
   cplx wall_bot[3], wall_top[3];
-  cplx* U = sc.U;
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    for (int j = 0; j < Nz; ++j) {
-      cplx r;
-      if (c == 0) r = (Dx * CP[(long)j * st] - fin(fin_, 0, j)) / B.eta;
-      else if (c == 1) r = (Dy * CP[(long)j * st] - fin(fin_, 1, j)) / B.eta;
-      else r = (DPv[(long)j * st] - fin(fin_, 2, j)) / B.eta;
-      if (j >= 2) r = r - (DM2(j - 2) * U[(long)(j - 2) * st]) / beta[(long)(j - 1) * st];
-      U[(long)j * st] = r;
-    }
-    U[(long)(Nz - 1) * st] = U[(long)(Nz - 1) * st] / beta[(long)Nz * st];
-    U[(long)(Nz - 2) * st] = U[(long)(Nz - 2) * st] / beta[(long)(Nz - 1) * st];
-    for (int i = Nz - 3; i >= 0; --i)
-      U[(long)i * st] = (U[(long)i * st] - DP2(i) * U[(long)(i + 2) * st]) / beta[(long)(i + 1) * st];
-    cplx ya = mk(0, 0), yb = mk(0, 0);
-    for (int j = 0; j < Nz; ++j) {
-      const cplx x = U[(long)j * st];
-      ya = ya + x * C0(j);
-      yb = yb + x * C1(j);
-    }
-    ya = ya - bc[2 * c];
-    yb = yb - bc[2 * c + 1];
-    const cplx Ya = Gi00 * ya + Gi01 * yb;
-    const cplx Yb = Gi10 * ya + Gi11 * yb;
-#define XU(j) ((j) < Nz ? (U[(long)(j) * st] - ((((j) & 1) ? ab[(long)(j) * st] * Yb : ab[(long)(j) * st] * Ya))) \
-                        : ((j) == Nz ? Ya : Yb))
+    const cplx* Uv = Uc[c];
+    const cplx ya = Ya[c], yb = Yb[c];
+#define XU(j) ((j) < Nz ? (Uv[(long)(j) * st] - ((((j) & 1) ? AB(j) * yb : AB(j) * ya))) : ((j) == Nz ? ya : yb))
     cplx sb = mk(0, 0), stp = mk(0, 0);
-    for (int m = 0; m < Nz; ++m) {
-      cplx cu = B.sd0[m] * XU(m);
-      if (m >= 2) cu = B.sm2[m] * XU(m - 2) + cu;
-      cu = cu + B.sp2[m] * XU(m + 2);
-      if (m == 0) cu = cu + B.S0N * Ya;
-      if (m == 1) cu = cu + B.S1N1 * Yb;
-      fout(c, m, cu);
-      stp = stp + cu;
-      sb = (m & 1) ? sb - cu : sb + cu;
+    cplx xm2 = mk(0, 0), x0 = XU(0), x1 = XU(1), x2 = XU(2), xm1 = mk(0, 0);
+    for (int m0 = 0; m0 < Nz; m0 += CH) {
+      cplx ahead[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int j = m0 + u + 3;
+        if (m0 + u < Nz && j <= Nz + 1) ahead[u] = XU(j);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int m = m0 + u;
+        if (m < Nz) {
+          cplx cu = B.sd0[m] * x0;
+          if (m >= 2) cu = B.sm2[m] * xm2 + cu;
+          cu = cu + B.sp2[m] * x2;
+          if (m == 0) cu = cu + B.S0N * ya;
+          if (m == 1) cu = cu + B.S1N1 * yb;
+          fout(c, m, cu);
+          stp = stp + cu;
+          sb = (m & 1) ? sb - cu : sb + cu;
+          xm2 = xm1; xm1 = x0; x0 = x1; x1 = x2;
+          if (m + 3 <= Nz + 1) x2 = ahead[u];
+        }
+      }
     }
 #undef XU
     wall_bot[c] = sb;
@@ -386,6 +592,9 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
 #undef DM2
 #undef C0
 #undef C1
+#undef IB
+#undef AB
+#undef PP
 
   // ---- wall correction coefficients ----
   for (int c = 0; c < 3; ++c)

@@ -23,12 +23,17 @@ struct StencilArgs {
   int rs_m, rs_d;        // record strides (elements)
   int* key;              // tile index
   int TX, TY, ntx;
+  const int* kt_m;       // z fold tables, to record the interior rows reached
+  const int* kt_d;
+  const double *fA_m, *fA_d, *fB_d;
+  int4* zrange;          // per particle: (klo, khi) monopole, (klo, khi) dipole
 };
 
 struct SpreadArgs {
   int Nx, Ny, Nz;
   int TX, TY, ZC, ntx, nty, RX, RY;
   const int *start, *order;
+  const int4* zrange;
   KernelDesc km, kd;
   const int4 *sup_m, *sup_d;
   const void *rec_m, *rec_d;
@@ -65,14 +70,15 @@ struct ZArgs {
 struct BvpArgs {
   BvpConsts B;   // pointers are device pointers
   int Nx, Ny, Nkx, Nm;
+  int Ms;                  // scratch stride: Nm + nspecial lanes
   double Lx, Ly;
-  const int* is_special;   // [Nm]
-  const int* special;      // list for the special launch (null for the main launch)
+  const int* special;      // modes touching a Nyquist index
   int nspecial;
-  double2* u2;             // [nspecial][3][Nz]
-  double *s_beta, *s_ab;   // [Nz+1][Nm], [Nz][Nm]
-  double2 *s_P, *s_CP, *s_DP, *s_U;   // [Nz][Nm] each
+  double2* u2;             // [nspecial][3][Nz] mirror solutions
+  double *s_ib, *s_ab;     // [Nz+1][Ms], [Nz][Ms]
+  double2 *s_P, *s_CP, *s_DP, *s_U0, *s_U1, *s_U2;   // [Nz][Ms] each
   double2* corr;           // [Nm][3][4]
+  double2* corr2;          // [nspecial][3][4]
 };
 
 template <class Tin, class T> void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s);

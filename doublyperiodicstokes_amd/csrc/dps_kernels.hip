@@ -8,7 +8,7 @@
 //                 the fly, one coalesced write per grid element
 //                 (SpreadInterp.cpp:302-448 + BoundaryConditions.h:22-361 +
 //                  SpreadInterpWrapper.cpp:66-107)
-//  k_interp       grid -> particles, 16 lanes per particle, derivative kernels
+//  k_interp       grid -> particles, one wavefront per particle, derivative kernels
 //                 for the angular velocity (SpreadInterp.cpp:450-600,
 //                 BoundaryConditions.h:377-731, FCM.py:411-426)
 //  k_zfwd/k_zinv  Chebyshev transform in z on even/odd folded node pairs
@@ -98,6 +98,24 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
         rec[o + 2 * W + q] = (T)d;
         rec[o + 3 * W + q] = (T)(d * qw);
       }
+    }
+    {
+      // interior rows reached through this support (after the wall fold)
+      int klo = 1 << 30, khi = -1;
+      const int* kt = kind ? A.kt_d : A.kt_m;
+      const double* fa = kind ? A.fA_d : A.fA_m;
+      const double* fb = kind ? A.fB_d : nullptr;
+      for (int q = 0; q < s.nz; ++q) {
+        const int e = s.e0 + q;
+        if (fa[e] != 0.0 || (kind && fb[e] != 0.0)) {
+          klo = min(klo, kt[e]);
+          khi = max(khi, kt[e]);
+        }
+      }
+      int4 zr = A.zrange[p];
+      if (kind) { zr.z = klo; zr.w = khi; }
+      else { zr.x = klo; zr.y = khi; zr.z = 1 << 30; zr.w = -1; }
+      A.zrange[p] = zr;
     }
     if (kind == 0) {
       // tile key from the monopole support centre
@@ -193,7 +211,11 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       for (int pi = pb; pi < pe; ++pi, ++job) {
         if ((job % nwaves) != wave) continue;
         const int p = A.order[pi];
+        const int4 zr = A.zrange[p];
         for (int kind = 0; kind < (Tq ? 2 : 1); ++kind) {
+          // interior z rows this particle touches
+          const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
+          if (khi < Z0 || klo >= Z0 + nZ) continue;
           const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
           const KernelDesc& K = kind ? A.kd : A.km;
           const int* kt = kind ? A.kt_d : A.kt_m;
@@ -218,34 +240,44 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               int iy0, ly0;
               const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
               if (!cy) continue;
-              const int tot = cx * cy * nzq;
-              for (int t = lane; t < tot; t += 64) {
-                const int a = t % cx;
-                const int r = t / cx;
-                const int b = r % cy;
-                const int q = r / cy;
-                const int e = sup.z + q;
-                const int k = kt[e] - Z0;
-                if (k < 0 || k >= nZ) continue;
+              // one lane per support column, walking the z support
+              for (int cb = 0; cb < cx * cy; cb += 64) {
+                const int cidx = cb + lane;
+                if (cidx >= cx * cy) continue;
+                const int a = cidx % cx, b = cidx / cx;
                 const int i = ix0 + a, j = iy0 + b;
-                T* dst = acc + k * plane + (ly0 + b) * A.TX + (lx0 + a);
+                T* col = acc + (ly0 + b) * A.TX + (lx0 + a);
                 if (kind == 0) {
-                  const T fa = (T)fA[e];
-                  if (fa == (T)0) continue;
-                  const T w = wx[i] * wy[j] * wz[q] * fa;
-                  atomicAdd(dst, w * f0);
-                  atomicAdd(dst + vol, w * f1);
-                  atomicAdd(dst + 2 * vol, w * f2);
+                  const T wxy = wx[i] * wy[j];
+                  for (int q = 0; q < nzq; ++q) {
+                    const int e = sup.z + q;
+                    const int k = kt[e] - Z0;
+                    if (k < 0 || k >= nZ) continue;
+                    const T fa = (T)fA[e];
+                    if (fa == (T)0) continue;
+                    const T w = wxy * wz[q] * fa;
+                    T* dst = col + k * plane;
+                    atomicAdd(dst, w * f0);
+                    atomicAdd(dst + vol, w * f1);
+                    atomicAdd(dst + 2 * vol, w * f2);
+                  }
                 } else {
-                  const T fa = (T)fA[e], fb = (T)fB[e];
-                  if (fa == (T)0 && fb == (T)0) continue;
-                  const T gx = dxw[i] * wy[j] * wz[q] * fa;
-                  const T gy = wx[i] * dyw[j] * wz[q] * fa;
-                  const T gz = wx[i] * wy[j] * dzw[q] * fb;
+                  const T gxy = dxw[i] * wy[j], gyx = wx[i] * dyw[j], wxy = wx[i] * wy[j];
                   const T h = (T)0.5;
-                  atomicAdd(dst, h * (gy * f2 - gz * f1));
-                  atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
-                  atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
+                  for (int q = 0; q < nzq; ++q) {
+                    const int e = sup.z + q;
+                    const int k = kt[e] - Z0;
+                    if (k < 0 || k >= nZ) continue;
+                    const T fa = (T)fA[e], fb = (T)fB[e];
+                    if (fa == (T)0 && fb == (T)0) continue;
+                    const T gx = gxy * wz[q] * fa;
+                    const T gy = gyx * wz[q] * fa;
+                    const T gz = wxy * dzw[q] * fb;
+                    T* dst = col + k * plane;
+                    atomicAdd(dst, h * (gy * f2 - gz * f1));
+                    atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
+                    atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
+                  }
                 }
               }
             }
@@ -256,12 +288,11 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   }
   __syncthreads();
   // one write per grid element of the tile
-  const int rowsz = nX;
   const int nrows = 3 * nZ * nY;
   for (int t = threadIdx.x; t < nrows * A.TX; t += blockDim.x) {
     const int ix = t % A.TX;
     const int row = t / A.TX;
-    if (ix >= rowsz) continue;
+    if (ix >= nX) continue;
     const int jy = row % nY;
     const int r2 = row / nY;
     const int kz = r2 % nZ;
@@ -272,15 +303,13 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
 }
 
 // ---------------------------------------------------------------------------
-// interpolation (16 lanes per particle)
+// interpolation: one wavefront per particle, one lane per support column
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ T group_sum16(T v)
+__device__ __forceinline__ T wave_sum(T v)
 {
-  v += __shfl_xor(v, 8, 16);
-  v += __shfl_xor(v, 4, 16);
-  v += __shfl_xor(v, 2, 16);
-  v += __shfl_xor(v, 1, 16);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
@@ -288,8 +317,8 @@ template <class T, class Tout>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__ vel,
                                                 Tout* __restrict__ ang)
 {
-  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-  const int l = threadIdx.x & 15;
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (g >= A.N) return;
   const int p = A.order[g];
   const T* __restrict__ U = reinterpret_cast<const T*>(A.grid);
@@ -302,25 +331,28 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     const T* wy = rec + K.wx;
     const T* wzi = rec + K.wx + K.wy + K.wzmax;
     T u0 = 0, u1 = 0, u2 = 0;
-    const int tot = K.wx * K.wy * sup.w;
-    for (int t = l; t < tot; t += 16) {
-      const int a = t % K.wx;
-      const int r = t / K.wx;
-      const int b = r % K.wy;
-      const int q = r / K.wy;
-      const int e = sup.z + q;
-      const T fa = (T)A.fA_m[e];
-      if (fa == (T)0) continue;
-      const size_t idx = ((size_t)A.kt_m[e] * A.Ny + wrapi(sup.y + b, A.Ny)) * A.Nx + wrapi(sup.x + a, A.Nx);
-      const T w = wx[a] * wy[b] * wzi[q] * fa;
-      u0 += w * U[idx];
-      u1 += w * U[idx + cs];
-      u2 += w * U[idx + 2 * cs];
+    for (int cb = 0; cb < K.wx * K.wy; cb += 64) {
+      const int cidx = cb + lane;
+      if (cidx < K.wx * K.wy) {
+        const int a = cidx % K.wx, b = cidx / K.wx;
+        const size_t col = (size_t)wrapi(sup.y + b, A.Ny) * A.Nx + wrapi(sup.x + a, A.Nx);
+        const T wxy = wx[a] * wy[b];
+        for (int q = 0; q < sup.w; ++q) {
+          const int e = sup.z + q;
+          const T fa = (T)A.fA_m[e];
+          if (fa == (T)0) continue;
+          const size_t idx = (size_t)A.kt_m[e] * A.Ny * A.Nx + col;
+          const T w = wxy * wzi[q] * fa;
+          u0 += w * U[idx];
+          u1 += w * U[idx + cs];
+          u2 += w * U[idx + 2 * cs];
+        }
+      }
     }
-    u0 = group_sum16(u0);
-    u1 = group_sum16(u1);
-    u2 = group_sum16(u2);
-    if (l == 0) {
+    u0 = wave_sum(u0);
+    u1 = wave_sum(u1);
+    u2 = wave_sum(u2);
+    if (lane == 0) {
       vel[3 * p] = (Tout)u0;
       vel[3 * p + 1] = (Tout)u1;
       vel[3 * p + 2] = (Tout)u2;
@@ -340,27 +372,30 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     T d[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = 0;
-    const int tot = K.wx * K.wy * sup.w;
-    for (int t = l; t < tot; t += 16) {
-      const int a = t % K.wx;
-      const int r = t / K.wx;
-      const int b = r % K.wy;
-      const int q = r / K.wy;
-      const int e = sup.z + q;
-      const T fa = (T)A.fA_d[e], fb = (T)A.fB_d[e];
-      if (fa == (T)0 && fb == (T)0) continue;
-      const size_t idx = ((size_t)A.kt_d[e] * A.Ny + wrapi(sup.y + b, A.Ny)) * A.Nx + wrapi(sup.x + a, A.Nx);
-      const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
-      const T gx = dxw[a] * wy[b] * wzi[q] * fa;
-      const T gy = wx[a] * dyw[b] * wzi[q] * fa;
-      const T gz = wx[a] * wy[b] * dzi[q] * fb;
-      d[0] += gx * v0; d[1] += gx * v1; d[2] += gx * v2;
-      d[3] += gy * v0; d[4] += gy * v1; d[5] += gy * v2;
-      d[6] += gz * v0; d[7] += gz * v1; d[8] += gz * v2;
+    for (int cb = 0; cb < K.wx * K.wy; cb += 64) {
+      const int cidx = cb + lane;
+      if (cidx < K.wx * K.wy) {
+        const int a = cidx % K.wx, b = cidx / K.wx;
+        const size_t col = (size_t)wrapi(sup.y + b, A.Ny) * A.Nx + wrapi(sup.x + a, A.Nx);
+        const T gxy = dxw[a] * wy[b], gyx = wx[a] * dyw[b], wxy = wx[a] * wy[b];
+        for (int q = 0; q < sup.w; ++q) {
+          const int e = sup.z + q;
+          const T fa = (T)A.fA_d[e], fb = (T)A.fB_d[e];
+          if (fa == (T)0 && fb == (T)0) continue;
+          const size_t idx = (size_t)A.kt_d[e] * A.Ny * A.Nx + col;
+          const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
+          const T gx = gxy * wzi[q] * fa;
+          const T gy = gyx * wzi[q] * fa;
+          const T gz = wxy * dzi[q] * fb;
+          d[0] += gx * v0; d[1] += gx * v1; d[2] += gx * v2;
+          d[3] += gy * v0; d[4] += gy * v1; d[5] += gy * v2;
+          d[6] += gz * v0; d[7] += gz * v1; d[8] += gz * v2;
+        }
+      }
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) d[i] = group_sum16(d[i]);
-    if (l == 0) {
+    for (int i = 0; i < 9; ++i) d[i] = wave_sum(d[i]);
+    if (lane == 0) {
       // omega = -1/2 (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)  (FCM.py:424-426)
       ang[3 * p] = (Tout)((T)(-0.5) * (d[5] - d[7]));
       ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[6] - d[2]));
@@ -407,21 +442,41 @@ __global__ void __launch_bounds__(256) k_zfwd(ZArgs A, const typename C2<T>::t* 
   __syncthreads();
   if (m >= A.Nm) return;
   V* dst = out + (size_t)c * N * A.Nm;
-  for (int k = threadIdx.x / CW; k < N; k += RS) {
-    const int kk = k >> 1;
-    const T* Tr = (k & 1) ? ((const T*)A.TfO + (size_t)kk * A.NH) : ((const T*)A.TfE + (size_t)kk * A.NH);
-    const V* X = (k & 1) ? O : E;
-    T sr = 0, si = 0;
-    for (int j = 0; j < A.NH; ++j) {
-      const T t = Tr[j];
-      const V x = X[j * CW + col];
-      sr += t * x.x;
-      si += t * x.y;
+  // RS is even, so all outputs of one thread share the parity of its row;
+  // each pass computes RB of them from one read of the folded column
+  constexpr int RB = 4;
+  const int row = threadIdx.x / CW;
+  const V* X = (row & 1) ? O : E;
+  const T* Tbase = (row & 1) ? (const T*)A.TfO : (const T*)A.TfE;
+  for (int kb = row; kb < N; kb += RS * RB) {
+    T sr[RB], si[RB];
+    const T* Tr[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int kr = kb + r * RS;
+      Tr[r] = Tbase + (size_t)((kr < N ? kr : kb) >> 1) * A.NH;
+      sr[r] = 0;
+      si[r] = 0;
     }
-    V r;
-    r.x = sr;
-    r.y = si;
-    dst[(size_t)k * A.Nm + m] = r;
+    for (int j = 0; j < A.NH; ++j) {
+      const V x = X[j * CW + col];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const T t = Tr[r][j];
+        sr[r] += t * x.x;
+        si[r] += t * x.y;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int kr = kb + r * RS;
+      if (kr < N) {
+        V v;
+        v.x = sr[r];
+        v.y = si[r];
+        dst[(size_t)kr * A.Nm + m] = v;
+      }
+    }
   }
 }
 
@@ -461,45 +516,65 @@ __global__ void __launch_bounds__(256) k_zinv(ZArgs A, const typename C2<T>::t* 
     if (m == 0) nb = 0;
   }
   V* dst = out + (size_t)c * N * A.Nm;
-  for (int j = threadIdx.x / CW; j < A.NH; j += RS) {
-    T ar = 0, ai = 0, br = 0, bi = 0;
-    const T* te = (const T*)A.TiE + (size_t)j * A.NE;
-    const T* to = (const T*)A.TiO + (size_t)j * A.NO;
+  constexpr int RB = 4;
+  for (int jb = threadIdx.x / CW; jb < A.NH; jb += RS * RB) {
+    T ar[RB], ai[RB], br[RB], bi[RB];
+    const T* te[RB];
+    const T* to[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int jr = jb + r * RS;
+      const int js = jr < A.NH ? jr : jb;
+      te[r] = (const T*)A.TiE + (size_t)js * A.NE;
+      to[r] = (const T*)A.TiO + (size_t)js * A.NO;
+      ar[r] = ai[r] = br[r] = bi[r] = 0;
+    }
     for (int kk = 0; kk < A.NE; ++kk) {
       const V x = Ck[(2 * kk) * CW + col];
-      ar += te[kk] * x.x;
-      ai += te[kk] * x.y;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        ar[r] += te[r][kk] * x.x;
+        ai[r] += te[r][kk] * x.y;
+      }
     }
     for (int kk = 0; kk < A.NO; ++kk) {
       const V x = Ck[(2 * kk + 1) * CW + col];
-      br += to[kk] * x.x;
-      bi += to[kk] * x.y;
-    }
-    const int j2 = N - 1 - j;
-    for (int side = 0; side < 2; ++side) {
-      const int jj = side ? j2 : j;
-      if (side && j2 == j) break;
-      T vr = side ? (ar - br) : (ar + br);
-      T vi = side ? (ai - bi) : (ai + bi);
-      if (nb) {
-        const double z = A.zpts[jj];
-        const double e1 = exp(-K * z);
-        double sr = cc[0].re * e1, si = cc[0].im * e1;
-        if (nb == 2) {
-          sr += cc[1].re * (z * e1);
-          si += cc[1].im * (z * e1);
-        } else {
-          const double e2 = exp(K * (z - A.Hs));
-          sr += cc[1].re * e2 + cc[2].re * (z * e1) + cc[3].re * (z * e2);
-          si += cc[1].im * e2 + cc[2].im * (z * e1) + cc[3].im * (z * e2);
-        }
-        vr += (T)(sr * A.inv_nxy);
-        vi += (T)(si * A.inv_nxy);
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        br[r] += to[r][kk] * x.x;
+        bi[r] += to[r][kk] * x.y;
       }
-      V r;
-      r.x = vr;
-      r.y = vi;
-      dst[(size_t)jj * A.Nm + m] = r;
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int j = jb + r * RS;
+      if (j >= A.NH) continue;
+      const int j2 = N - 1 - j;
+      for (int side = 0; side < 2; ++side) {
+        const int jj = side ? j2 : j;
+        if (side && j2 == j) break;
+        T vr = side ? (ar[r] - br[r]) : (ar[r] + br[r]);
+        T vi = side ? (ai[r] - bi[r]) : (ai[r] + bi[r]);
+        if (nb) {
+          const double z = A.zpts[jj];
+          const double e1 = exp(-K * z);
+          double sr = cc[0].re * e1, si = cc[0].im * e1;
+          if (nb == 2) {
+            sr += cc[1].re * (z * e1);
+            si += cc[1].im * (z * e1);
+          } else {
+            const double e2 = exp(K * (z - A.Hs));
+            sr += cc[1].re * e2 + cc[2].re * (z * e1) + cc[3].re * (z * e2);
+            si += cc[1].im * e2 + cc[2].im * (z * e1) + cc[3].im * (z * e2);
+          }
+          vr += (T)(sr * A.inv_nxy);
+          vi += (T)(si * A.inv_nxy);
+        }
+        V v;
+        v.x = vr;
+        v.y = vi;
+        dst[(size_t)jj * A.Nm + m] = v;
+      }
     }
   }
 }
@@ -548,9 +623,8 @@ __device__ __forceinline__ double kvec(int i, int n, double L)
   return 2 * M_PI * (double)nn / L;
 }
 
-__device__ void store_corr(const BvpArgs& A, int m, const ModeResult& r)
+__device__ void store_corr(double2* cr, const ModeResult& r)
 {
-  double2* cr = A.corr + (size_t)m * 12;
   for (int c = 0; c < 3; ++c)
     for (int b = 0; b < 4; ++b) {
       double2 v;
@@ -564,78 +638,88 @@ template <class T>
 __global__ void __launch_bounds__(64) k_bvp(BvpArgs A, const typename C2<T>::t* __restrict__ fin,
                                             typename C2<T>::t* __restrict__ fout)
 {
+  // lanes [0, Nm): every mode of the half spectrum; lanes [Nm, Nm + nspecial):
+  // the mirror solve of each mode that touches a Nyquist index
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int m;
-  if (A.special) {
-    if (t >= A.nspecial) return;
-    m = A.special[t];
-  } else {
-    m = t;
-    if (m >= A.Nm) return;
-    if (A.is_special[m]) return;
-  }
+  if (t >= A.Nm + A.nspecial) return;
+  const bool mirror = t >= A.Nm;
+  const int s = t - A.Nm;
+  const int m = mirror ? A.special[s] : t;
   const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
   CoefIn<T> in;
   in.p = fin + m;
   in.cs = (long)A.B.Nz * A.Nm;
   in.ks = A.Nm;
-  in.conj = false;
-  CoefOut<T> out;
-  out.p = fout + m;
-  out.cs = in.cs;
-  out.ks = in.ks;
+  in.conj = mirror;
   ScratchView sc;
-  sc.stride = A.Nm;
-  sc.beta = A.s_beta + m;
-  sc.ab = A.s_ab + m;
-  sc.P = reinterpret_cast<cplx*>(A.s_P) + m;
-  sc.CP = reinterpret_cast<cplx*>(A.s_CP) + m;
-  sc.DP = reinterpret_cast<cplx*>(A.s_DP) + m;
-  sc.U = reinterpret_cast<cplx*>(A.s_U) + m;
-  ModeResult r1;
-  if (m == 0) {
-    bvp_k0(A.B, in, out, sc.U, sc.stride);
-    for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < 4; ++b) r1.corr[c][b] = mk(0, 0);
-    store_corr(A, 0, r1);
-    return;
+  sc.stride = A.Ms;
+  sc.ib = A.s_ib + t;
+  sc.ab = A.s_ab + t;
+  sc.P = reinterpret_cast<cplx*>(A.s_P) + t;
+  sc.CP = reinterpret_cast<cplx*>(A.s_CP) + t;
+  sc.DP = reinterpret_cast<cplx*>(A.s_DP) + t;
+  sc.U0 = reinterpret_cast<cplx*>(A.s_U0) + t;
+  sc.U1 = reinterpret_cast<cplx*>(A.s_U1) + t;
+  sc.U2 = reinterpret_cast<cplx*>(A.s_U2) + t;
+  ModeResult r;
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
+  r.nbasis = 0;
+  if (!mirror) {
+    CoefOut<T> out;
+    out.p = fout + m;
+    out.cs = in.cs;
+    out.ks = in.ks;
+    if (m == 0) bvp_k0(A.B, in, out, sc.U0, sc.stride);
+    else {
+      const double kx = kvec(ix, A.Nx, A.Lx), ky = kvec(iy, A.Ny, A.Ly);
+      const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
+      const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+      bvp_mode(A.B, kx, ky, nyqx, nyqy, in, out, sc, r);
+    }
+    store_corr(A.corr + (size_t)m * 12, r);
+  } else {
+    // the reference's c2c transform keeps only the real part of the inverse,
+    // i.e. the Hermitian average of a mode and its mirror solved with the
+    // mirror's own grid wavenumbers; this lane solves the mirror
+    const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
+    const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
+    const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
+    const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
+    CplxOut o2;
+    o2.p = A.u2 + (size_t)s * 3 * A.B.Nz;
+    o2.cs = A.B.Nz;
+    o2.ks = 1;
+    bvp_mode(A.B, kxm, kym, nyqxm, nyqym, in, o2, sc, r);
+    store_corr(A.corr2 + (size_t)s * 12, r);
   }
-  const double kx = kvec(ix, A.Nx, A.Lx), ky = kvec(iy, A.Ny, A.Ly);
-  const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
-  const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
-  bvp_mode(A.B, kx, ky, nyqx, nyqy, in, out, sc, r1);
-  if (!A.special) {
-    store_corr(A, m, r1);
-    return;
-  }
-  // mode touching a Nyquist index: the reference's c2c transform keeps only
-  // the real part of the inverse, i.e. the Hermitian average of this mode and
-  // its mirror solved with the mirror's own (grid) wavenumbers.
-  const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
-  const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
-  const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
-  const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
-  in.conj = true;
-  CplxOut o2;
-  o2.p = A.u2 + (size_t)t * 3 * A.B.Nz;
-  o2.cs = A.B.Nz;
-  o2.ks = 1;
-  ModeResult r2;
-  bvp_mode(A.B, kxm, kym, nyqxm, nyqym, in, o2, sc, r2);
+}
+
+// average each Nyquist-touching mode with its mirror solution
+template <class T>
+__global__ void k_bvp_combine(BvpArgs A, typename C2<T>::t* __restrict__ fout)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.nspecial) return;
+  const int m = A.special[s];
+  const long cs = (long)A.B.Nz * A.Nm;
   for (int c = 0; c < 3; ++c)
     for (int k = 0; k < A.B.Nz; ++k) {
-      typename C2<T>::t* q = fout + (size_t)c * in.cs + (size_t)k * in.ks + m;
-      const double2 v2 = o2.p[c * o2.cs + k];
+      typename C2<T>::t* q = fout + c * cs + (long)k * A.Nm + m;
+      const double2 v2 = A.u2[((size_t)s * 3 + c) * A.B.Nz + k];
       const double re = 0.5 * ((double)q->x + v2.x);
       const double im = 0.5 * ((double)q->y - v2.y);
       q->x = (T)re;
       q->y = (T)im;
     }
-  for (int c = 0; c < 3; ++c)
-    for (int b = 0; b < 4; ++b)
-      r1.corr[c][b] = mk(0.5 * (r1.corr[c][b].re + r2.corr[c][b].re),
-                         0.5 * (r1.corr[c][b].im - r2.corr[c][b].im));
-  store_corr(A, m, r1);
+  double2* c1 = A.corr + (size_t)m * 12;
+  const double2* c2 = A.corr2 + (size_t)s * 12;
+  for (int i = 0; i < 12; ++i) {
+    double2 v;
+    v.x = 0.5 * (c1[i].x + c2[i].x);
+    v.y = 0.5 * (c1[i].y - c2[i].y);
+    c1[i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -713,8 +797,8 @@ template <class T, class Tout>
 void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s)
 {
   if (A.N == 0) return;
-  const int groups_per_block = 256 / 16;
-  hipLaunchKernelGGL((k_interp<T, Tout>), dim3((A.N + groups_per_block - 1) / groups_per_block),
+  const int waves_per_block = 256 / 64;
+  hipLaunchKernelGGL((k_interp<T, Tout>), dim3((A.N + waves_per_block - 1) / waves_per_block),
                      dim3(256), 0, s, A, (Tout*)vel, (Tout*)ang);
 }
 
@@ -755,9 +839,10 @@ template <class T>
 void launch_bvp(const BvpArgs& A, const void* in, void* out, hipStream_t s)
 {
   typedef typename C2<T>::t V;
-  const int n = A.special ? A.nspecial : A.Nm;
-  if (n == 0) return;
+  const int n = A.Nm + A.nspecial;
   hipLaunchKernelGGL((k_bvp<T>), dim3((n + 63) / 64), dim3(64), 0, s, A, (const V*)in, (V*)out);
+  if (A.nspecial > 0)
+    hipLaunchKernelGGL((k_bvp_combine<T>), dim3((A.nspecial + 63) / 64), dim3(64), 0, s, A, (V*)out);
 }
 
 template <class T>

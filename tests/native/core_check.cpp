@@ -115,18 +115,18 @@ void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const 
   B.S0N = T.S0N; B.S1N1 = T.S1N1;
   B.fm1 = T.fm1.data(); B.fp1 = T.fp1.data(); B.F0N1 = T.F0N1;
   B.R1 = T.R1.data(); B.R2 = T.R2.data(); B.L1 = T.L1.data(); B.L2 = T.L2.data();
-  std::vector<double> beta(Nz + 1), ab(Nz);
-  std::vector<cplx> P(Nz), CP(Nz), DP(Nz), U(Nz);
+  std::vector<double> ib(Nz + 1), ab(Nz);
+  std::vector<cplx> P(Nz), CP(Nz), DP(Nz), U0(Nz), U1(Nz), U2(Nz);
   ScratchView sc;
-  sc.beta = beta.data(); sc.ab = ab.data(); sc.P = P.data(); sc.CP = CP.data(); sc.DP = DP.data();
-  sc.U = U.data(); sc.stride = 1;
+  sc.ib = ib.data(); sc.ab = ab.data(); sc.P = P.data(); sc.CP = CP.data(); sc.DP = DP.data();
+  sc.U0 = U0.data(); sc.U1 = U1.data(); sc.U2 = U2.data(); sc.stride = 1;
   for (int m = 0; m < M; ++m) {
     HostIn in{fin + (size_t)m * 6 * Nz, Nz, false};
     HostOut out{uout + (size_t)m * 6 * Nz, Nz};
     ModeResult r;
     for (int c = 0; c < 3; ++c)
       for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
-    if (kx[m] == 0 && ky[m] == 0) bvp_k0(B, in, out, U.data(), 1);
+    if (kx[m] == 0 && ky[m] == 0) bvp_k0(B, in, out, U0.data(), 1);
     else bvp_mode(B, kx[m], ky[m], nyqx[m], nyqy[m], in, out, sc, r);
     for (int c = 0; c < 3; ++c)
       for (int b = 0; b < 4; ++b) {
