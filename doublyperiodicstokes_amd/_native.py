@@ -21,6 +21,8 @@ DPS_MODE_BOTTOM = 2
 DPS_MODE_SLIT = 3
 DPS_PREC_DOUBLE = 0
 DPS_PREC_MIXED = 1
+DPS_SOLVER_AUTO = 0
+DPS_SOLVER_LANE = 1
 DPS_F32 = 0
 DPS_F64 = 1
 
@@ -37,7 +39,8 @@ class DpsParams(ctypes.Structure):
                 ("alpha", ctypes.c_double), ("alpha_d", ctypes.c_double),
                 ("beta", ctypes.c_double), ("beta_d", ctypes.c_double),
                 ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
-                ("mode", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("precision", ctypes.c_int),
+                ("solver", ctypes.c_int)]
 
 
 class DpsError(RuntimeError):

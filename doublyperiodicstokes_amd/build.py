@@ -14,8 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libdpstokes_hip.so"
-SOURCES = ["dps_api.hip", "dps_kernels.hip"]
-HEADERS = ["dps_core.h", "dps_internal.h", "dps_tables.h"]
+SOURCES = ["dps_api.hip", "dps_kernels.hip", "dps_spectral.hip"]
+HEADERS = ["dps_core.h", "dps_internal.h", "dps_seg.h", "dps_tables.h"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
@@ -30,7 +30,7 @@ def _hipcc():
     return p
 
 
-def build(verbose=False, jobs=2):
+def build(verbose=False):
     """Compile every source to an object, then link the shared library."""
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "_obj")

@@ -89,6 +89,11 @@ struct dps_handle {
   double2 *s_P = nullptr, *s_CP = nullptr, *s_DP = nullptr, *s_U0 = nullptr, *s_U1 = nullptr, *s_U2 = nullptr;
   double2 *d_corr = nullptr, *d_corr2 = nullptr;
   double* d_stage = nullptr;   // FP64 reference-layout staging for stage entry points
+  // fused spectral solve (lane groups)
+  bool fused = false;
+  int sp_lpm = 0, sp_e = 0, sp_G = 0;
+  void *d_TfP = nullptr, *d_TiP = nullptr;
+  double *d_fib = nullptr, *d_fab = nullptr, *d_fG = nullptr;
 
   rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
   rocfft_execution_info info = nullptr;
@@ -205,7 +210,10 @@ int build(dps_handle* h)
   }
   h->cheb = build_cheb_tables(h->Nz, h->H);
   h->ct = build_cheb_transform(h->Nz);
-  {
+  h->fused = (P.solver == DPS_SOLVER_AUTO) && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
+  if (h->fused) {
+    h->sp_G = spectral_modes_per_block(h->Nz, h->sp_lpm, h->prec == DPS_PREC_MIXED);
+  } else {
     ZArgs Z;
     Z.N = h->Nz;
     Z.NH = h->ct.NH;
@@ -231,6 +239,8 @@ std::vector<T> cast_vec(const std::vector<double>& v)
   for (size_t i = 0; i < v.size(); ++i) r[i] = (T)v[i];
   return r;
 }
+
+int allocate_fused(dps_handle* h);
 
 int allocate(dps_handle* h)
 {
@@ -311,6 +321,7 @@ int allocate(dps_handle* h)
   if ((rc = h->upload(&h->d_R2, h->cheb.R2))) return rc;
   if ((rc = h->upload(&h->d_L1, h->cheb.L1))) return rc;
   if ((rc = h->upload(&h->d_L2, h->cheb.L2))) return rc;
+  if (h->fused) return allocate_fused(h);
   // modes that touch a Nyquist index (their mirror is solved separately)
   std::vector<int> sp;
   for (int iy = 0; iy < h->Ny; ++iy)
@@ -333,6 +344,66 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc(&h->s_U1, (size_t)h->Nz * ms))) return rc;
   if ((rc = h->alloc(&h->s_U2, (size_t)h->Nz * ms))) return rc;
   if ((rc = h->alloc(&h->d_corr, (size_t)h->Nm * 12))) return rc;
+  return DPS_OK;
+}
+
+BvpArgs bvp_args(dps_handle* h);
+
+// Tables of the fused spectral kernel: transform rows in lane order and the
+// static per-mode factorisation (computed once on the device).
+int allocate_fused(dps_handle* h)
+{
+  int rc;
+  const int Nz = h->Nz, NH = h->ct.NH, LPM = h->sp_lpm, E = h->sp_e, R = E / 2;
+  const double inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
+  std::vector<double> tf((size_t)NH * E * LPM, 0.0), ti((size_t)Nz * R * LPM, 0.0);
+  for (int j = 0; j < NH; ++j)
+    for (int q = 0; q < E; ++q)
+      for (int l = 0; l < LPM; ++l) {
+        const int k = l * E + q;
+        if (k >= Nz) continue;
+        tf[((size_t)j * E + q) * LPM + l] =
+            (k & 1) ? h->ct.TfO[(size_t)(k / 2) * NH + j] : h->ct.TfE[(size_t)(k / 2) * NH + j];
+      }
+  for (int k = 0; k < Nz; ++k)
+    for (int r = 0; r < R; ++r)
+      for (int l = 0; l < LPM; ++l) {
+        const int j = l + LPM * r;
+        if (j >= NH) continue;
+        ti[((size_t)k * R + r) * LPM + l] =
+            inv_nxy * ((k & 1) ? h->ct.TiO[(size_t)j * h->ct.NO + k / 2] : h->ct.TiE[(size_t)j * h->ct.NE + k / 2]);
+      }
+  if (h->prec == DPS_PREC_MIXED) {
+    if ((rc = h->upload((float**)&h->d_TfP, cast_vec<float>(tf)))) return rc;
+    if ((rc = h->upload((float**)&h->d_TiP, cast_vec<float>(ti)))) return rc;
+  } else {
+    if ((rc = h->upload((double**)&h->d_TfP, tf))) return rc;
+    if ((rc = h->upload((double**)&h->d_TiP, ti))) return rc;
+  }
+  const size_t nf = (size_t)h->Nm * E * LPM;
+  if ((rc = h->alloc(&h->d_fib, nf))) return rc;
+  if ((rc = h->alloc(&h->d_fab, nf))) return rc;
+  if ((rc = h->alloc(&h->d_fG, (size_t)h->Nm * 4))) return rc;
+  double *t_ib = nullptr, *t_ab = nullptr;
+  if (hipMalloc((void**)&t_ib, (size_t)(Nz + 1) * h->Nm * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&t_ab, (size_t)Nz * h->Nm * sizeof(double)) != hipSuccess) {
+    if (t_ib) (void)hipFree(t_ib);
+    return fail(DPS_ERR_ALLOC, "hipMalloc of the factorisation scratch failed");
+  }
+  const BvpArgs A = bvp_args(h);
+  launch_bvp_factor(A.B, h->Nx, h->Ny, h->Nkx, h->Nm, h->P.Lx, h->P.Ly, t_ib, t_ab, LPM, E, h->d_fib,
+                    h->d_fab, h->d_fG, 0);
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(0);
+  (void)hipFree(t_ib);
+  (void)hipFree(t_ab);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(DPS_ERR_HIP, "factorisation kernel failed");
+  return DPS_OK;
+}
+
+int make_plans(dps_handle* h)
+{
+  int rc;
   // rocFFT plans: 2-D over (x, y), batched over (component, z)
   const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
   size_t lengths[2] = {(size_t)h->Nx, (size_t)h->Ny};
@@ -471,6 +542,22 @@ BvpArgs bvp_args(dps_handle* h)
   return A;
 }
 
+SpecArgs spec_args(dps_handle* h)
+{
+  SpecArgs A;
+  A.B = bvp_args(h).B;
+  A.Nx = h->Nx; A.Ny = h->Ny; A.Nkx = h->Nkx; A.Nm = h->Nm;
+  A.Lx = h->P.Lx; A.Ly = h->P.Ly;
+  A.NH = h->ct.NH;
+  A.lpm = h->sp_lpm; A.e = h->sp_e; A.G = h->sp_G;
+  A.TfP = h->d_TfP; A.TiP = h->d_TiP;
+  A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
+  A.zpts = h->d_zpts;
+  A.Hs = h->Lz;
+  A.inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
+  return A;
+}
+
 void mark(dps_handle* h, int i, hipStream_t s)
 {
   if (h->prof) (void)hipEventRecord(h->ev[i], s);
@@ -496,6 +583,19 @@ int do_solve(dps_handle* h, hipStream_t s, int ev0)
   int rc;
   if ((rc = run_fft(h, true, h->d_grid, h->d_specA, s))) return rc;
   mark(h, ev0 + 1, s);
+  if (h->fused) {
+    // z transforms are inside the fused launch: their stage times read zero
+    mark(h, ev0 + 2, s);
+    const SpecArgs A = spec_args(h);
+    if (mixed(h)) launch_spectral<float>(A, h->d_specA, h->d_specB, s);
+    else launch_spectral<double>(A, h->d_specA, h->d_specB, s);
+    HIPCHK(hipGetLastError());
+    mark(h, ev0 + 3, s);
+    mark(h, ev0 + 4, s);
+    if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+    mark(h, ev0 + 5, s);
+    return DPS_OK;
+  }
   const ZArgs Z = z_args(h);
   if (mixed(h)) launch_zfwd<float>(Z, h->d_specA, h->d_specB, s);
   else launch_zfwd<double>(Z, h->d_specA, h->d_specB, s);
@@ -553,6 +653,8 @@ int dps_create(const dps_params* params, int number_particles, dps_handle** out)
     return fail(DPS_ERR_ARG, "mode must be nowall, bottom or slit");
   if (P.precision != DPS_PREC_DOUBLE && P.precision != DPS_PREC_MIXED)
     return fail(DPS_ERR_ARG, "precision must be DPS_PREC_DOUBLE or DPS_PREC_MIXED");
+  if (P.solver != DPS_SOLVER_AUTO && P.solver != DPS_SOLVER_LANE)
+    return fail(DPS_ERR_ARG, "solver must be DPS_SOLVER_AUTO or DPS_SOLVER_LANE");
   if (number_particles < 0) return fail(DPS_ERR_ARG, "number of particles must be >= 0");
   if (!(P.Lx > 0) || !(P.Ly > 0) || !(P.zmax > P.zmin) || !(P.viscosity > 0))
     return fail(DPS_ERR_ARG, "box lengths, z extent and viscosity must be positive");
@@ -567,6 +669,7 @@ int dps_create(const dps_params* params, int number_particles, dps_handle** out)
     if (g_rocfft_users++ == 0) rocfft_setup();
     fft_user = true;
     rc = allocate(h);
+    if (!rc) rc = make_plans(h);
   }
   if (rc) {
     const std::string msg = g_err;

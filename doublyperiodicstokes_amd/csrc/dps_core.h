@@ -216,24 +216,30 @@ struct ModeResult {
 // one latency per recurrence step.
 #define DPS_CH 8
 
-template <class In, class Out>
-DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int nyq_y,
-                     const In& fin_, Out& fout, const ScratchView& sc, ModeResult& res)
-{
-  const int Nz = B.Nz;
-  const long st = sc.stride;
-  const double H = B.H;
-  const double Ksq = kx * kx + ky * ky;
-  const double K = sqrt(Ksq);
-  const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
-  const cplx Dy = nyq_y ? mk(0, 0) : mk(0, ky);
-  constexpr int CH = DPS_CH;
+// Band entries of I - Ksq S and the Robin BC rows; the functions below define
+// Ksq, H, H2K, st, B and sc before using them.
 #define D0(i) (1.0 - B.sd0[(i)] * Ksq)
 #define DP2(i) (0.0 - B.sp2[(i)] * Ksq)
 #define DM2(i) (0.0 - B.sm2[(i) + 2] * Ksq)
 #define IB(i) sc.ib[(long)(i) * st]
 #define AB(i) sc.ab[(long)(i) * st]
 #define PP(i) sc.P[(long)(i) * st]
+// Robin BC rows H^2 K BCR2 + H BCR1 and H^2 K BCL2 + H BCL1 (Solvers.py:309-313)
+#define C0(j) (B.R2[(j)] * H2K + (B.R2[(j)] + (H * B.R1[(j)] - B.R2[(j)])))
+#define C1(j) (B.L2[(j)] * H2K + (-B.L2[(j)] + (H * B.L1[(j)] + B.L2[(j)])))
+
+// Factorisation that depends on |k| only: reciprocal KBPENTA pivots IB(1..Nz)
+// (LinearSolvers.h:52-74), A^{-1}B (LinearSolvers.cpp:42) on even/odd rows of
+// sc.ab, and the inverse of the 2x2 Schur complement G = C A^{-1}B - D
+// (LinearSolvers.cpp:44-56) returned as Gi = {Gi00, Gi01, Gi10, Gi11}.
+DPS_HD void bvp_factor(const BvpConsts& B, double Ksq, const ScratchView& sc, double* Gi)
+{
+  const int Nz = B.Nz;
+  const long st = sc.stride;
+  const double H = B.H;
+  const double K = sqrt(Ksq);
+  const double H2K = (H * H) * K;
+  constexpr int CH = DPS_CH;
   // KBPENTA pivots (LinearSolvers.h:52-74), stored as reciprocals
   {
     double bm2 = D0(0), bm1 = D0(1);
@@ -246,10 +252,6 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
       bm1 = b;
     }
   }
-  // Robin BC rows H^2 K BCR2 + H BCR1 and H^2 K BCL2 + H BCL1 (Solvers.py:309-313)
-  const double H2K = (H * H) * K;
-#define C0(j) (B.R2[(j)] * H2K + (B.R2[(j)] + (H * B.R1[(j)] - B.R2[(j)])))
-#define C1(j) (B.L2[(j)] * H2K + (-B.L2[(j)] + (H * B.L1[(j)] + B.L2[(j)])))
   // A^{-1}B (LinearSolvers.cpp:42): B = -Ksq SIMat[:, Nz:Nz+2] has one non-zero
   // per column (rows 0 and 1), so the two solutions live on even and odd rows.
   // G = C A^{-1}B - D and its explicit inverse (LinearSolvers.cpp:44-56).
@@ -303,7 +305,34 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
   }
   G00 -= C0(Nz); G10 -= C1(Nz); G01 -= C0(Nz + 1); G11 -= C1(Nz + 1);
   const double det = 1.0 / (G00 * G11 - G01 * G10);
-  const double Gi00 = G11 * det, Gi10 = -G10 * det, Gi01 = -G01 * det, Gi11 = G00 * det;
+  Gi[0] = G11 * det;
+  Gi[1] = -G01 * det;
+  Gi[2] = -G10 * det;
+  Gi[3] = G00 * det;
+}
+
+// Wall correction (DPTools.cpp:590-877) as coefficients of the analytic
+// profiles, from the no-wall velocity at the bottom (wall_bot) and top
+// (wall_top) boundaries of this mode.
+DPS_HD void wall_correction(const BvpConsts& B, double kx, double ky, const cplx* wall_bot,
+                            const cplx* wall_top, ModeResult& res);
+
+template <class In, class Out>
+DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int nyq_y,
+                     const In& fin_, Out& fout, const ScratchView& sc, ModeResult& res)
+{
+  const int Nz = B.Nz;
+  const long st = sc.stride;
+  const double H = B.H;
+  const double Ksq = kx * kx + ky * ky;
+  const double K = sqrt(Ksq);
+  const double H2K = (H * H) * K;
+  const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
+  const cplx Dy = nyq_y ? mk(0, 0) : mk(0, ky);
+  constexpr int CH = DPS_CH;
+  double Gi[4];
+  bvp_factor(B, Ksq, sc, Gi);
+  const double Gi00 = Gi[0], Gi01 = Gi[1], Gi10 = Gi[2], Gi11 = Gi[3];
 
   // ---- pressure: chebCoeffDiff + assemble_p_rhs (DPTools.cpp:134-195) ----
   {
@@ -587,6 +616,8 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
     wall_bot[c] = sb;
     wall_top[c] = stp;
   }
+  wall_correction(B, kx, ky, wall_bot, wall_top, res);
+}
 #undef D0
 #undef DP2
 #undef DM2
@@ -596,7 +627,10 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
 #undef AB
 #undef PP
 
-  // ---- wall correction coefficients ----
+DPS_HD void wall_correction(const BvpConsts& B, double kx, double ky, const cplx* wall_bot,
+                            const cplx* wall_top, ModeResult& res)
+{
+  const double K = sqrt(kx * kx + ky * ky);
   for (int c = 0; c < 3; ++c)
     for (int b = 0; b < 4; ++b) res.corr[c][b] = mk(0, 0);
   res.nbasis = 0;

@@ -8,6 +8,17 @@
 
 namespace dps {
 
+template <class T> struct C2;
+template <> struct C2<float> { typedef float2 t; };
+template <> struct C2<double> { typedef double2 t; };
+
+// wavenumber of FFT index i on n points over length L (Nyquist positive)
+__host__ __device__ inline double kvec(int i, int n, double L)
+{
+  const int nn = (i <= n / 2) ? i : i - n;
+  return 2 * M_PI * (double)nn / L;
+}
+
 struct StencilArgs {
   int N;
   double xmin, ymin, zmin;
@@ -91,6 +102,29 @@ size_t z_lds_bytes(const ZArgs& A, bool mixed);
 template <class T> void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_zinv(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_bvp(const BvpArgs& A, const void* in, void* out, hipStream_t s);
+// Fused spectral solve: z transform -> per-mode BVP on lane groups ->
+// Nyquist averaging -> inverse z transform with wall corrections.
+struct SpecArgs {
+  BvpConsts B;   // device pointers
+  int Nx, Ny, Nkx, Nm;
+  double Lx, Ly;
+  int NH;                 // node pairs ceil(Nz/2)
+  int lpm, e, G;          // lanes per mode, rows per lane, modes per block
+  const void* TfP;        // [NH][E][LPM] forward rows in lane order (grid precision)
+  const void* TiP;        // [Nz][E/2][LPM] inverse rows in lane order (scaled by 1/(Nx Ny))
+  const double* fac_ib;   // [Nm][E][LPM] reciprocal pivots of each lane's rows
+  const double* fac_ab;   // [Nm][E][LPM] A^{-1}B of each lane's rows
+  const double* fac_G;    // [Nm][4] inverse Schur complement
+  const double* zpts;
+  double Hs, inv_nxy;
+};
+// lane-group shape for Nz rows: returns false when Nz is too large
+bool spectral_shape(int Nz, int* lpm, int* e);
+int spectral_modes_per_block(int Nz, int lpm, bool mixed);
+void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
+                       double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,
+                       hipStream_t s);
+template <class T> void launch_spectral(const SpecArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_ref_to_grid(const double* ref, void* g, int Nx, int Ny, int Nz, hipStream_t s);
 template <class T> void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipStream_t s);
 

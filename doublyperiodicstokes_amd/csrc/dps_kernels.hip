@@ -26,10 +26,6 @@
 
 namespace dps {
 
-template <class T> struct C2;
-template <> struct C2<float> { typedef float2 t; };
-template <> struct C2<double> { typedef double2 t; };
-
 __device__ __forceinline__ int wrapi(int i, int n)
 {
   int r = i % n;
@@ -616,12 +612,6 @@ struct CplxOut {
     p[c * cs + k * ks] = r;
   }
 };
-
-__device__ __forceinline__ double kvec(int i, int n, double L)
-{
-  const int nn = (i <= n / 2) ? i : i - n;
-  return 2 * M_PI * (double)nn / L;
-}
 
 __device__ void store_corr(double2* cr, const ModeResult& r)
 {

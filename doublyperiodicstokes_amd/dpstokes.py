@@ -38,6 +38,7 @@ import numpy as np
 from . import _native as N
 
 _MODES = {"nowall": N.DPS_MODE_NOWALL, "bottom": N.DPS_MODE_BOTTOM, "slit": N.DPS_MODE_SLIT}
+_SOLVERS = {"auto": N.DPS_SOLVER_AUTO, "lane": N.DPS_SOLVER_LANE}
 
 
 def getPrecision():
@@ -49,12 +50,14 @@ class StokesParameters:
     """Parameters of a doubly periodic Stokes solve (uammd.StokesParameters).
 
     ``beta``/``beta_d`` are the ES exponents beta*w; ``<= 0`` selects 1.785*w.
-    ``mode`` is 'nowall', 'bottom' or 'slit'.
+    ``mode`` is 'nowall', 'bottom' or 'slit'.  ``solver`` picks the spectral
+    solve: 'auto' (fused lane-group kernel when nz <= 512) or 'lane' (one
+    wavenumber per lane, separate z-transform kernels).
     """
 
     def __init__(self, viscosity=1.0, Lx=0.0, Ly=0.0, zmin=0.0, zmax=0.0, w=1, w_d=1,
                  alpha=-1.0, alpha_d=-1.0, beta=-1.0, beta_d=-1.0, nx=-1, ny=-1, nz=-1,
-                 mode="none", precision=None):
+                 mode="none", precision=None, solver="auto"):
         self.viscosity = viscosity
         self.Lx, self.Ly = Lx, Ly
         self.zmin, self.zmax = zmin, zmax
@@ -65,6 +68,7 @@ class StokesParameters:
         self.mode = mode
         self.precision = precision if precision is not None else (
             "mixed" if getPrecision() == "single" else "double")
+        self.solver = solver
 
     def __str__(self):
         return ("viscosity: %g\nLx: %g\nLy: %g\nzmin: %g\nzmax: %g\nw: %g\nw_d: %g\nalpha: %g\n"
@@ -78,6 +82,8 @@ class StokesParameters:
             raise ValueError("mode must be one of 'nowall', 'bottom', 'slit' (got %r)" % (self.mode,))
         if self.precision not in ("double", "mixed"):
             raise ValueError("precision must be 'double' or 'mixed'")
+        if self.solver not in _SOLVERS:
+            raise ValueError("solver must be 'auto' or 'lane'")
         p = N.DpsParams()
         p.viscosity = float(self.viscosity)
         p.Lx, p.Ly = float(self.Lx), float(self.Ly)
@@ -86,6 +92,7 @@ class StokesParameters:
         p.alpha, p.alpha_d = float(self.alpha), float(self.alpha_d)
         p.beta, p.beta_d = float(self.beta), float(self.beta_d)
         p.nx, p.ny, p.nz = int(self.nx), int(self.ny), int(self.nz)
+        p.solver = _SOLVERS[self.solver]
         p.mode = _MODES[self.mode]
         p.precision = N.DPS_PREC_MIXED if self.precision == "mixed" else N.DPS_PREC_DOUBLE
         return p

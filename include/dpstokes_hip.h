@@ -46,6 +46,10 @@ extern "C" {
 #define DPS_PREC_DOUBLE 0       /* everything in FP64 */
 #define DPS_PREC_MIXED 1        /* FP32 grids/FFT/z-transform, FP64 BVP + corrections */
 
+/* spectral solve */
+#define DPS_SOLVER_AUTO 0       /* fused lane-group kernel when nz <= 512, else DPS_SOLVER_LANE */
+#define DPS_SOLVER_LANE 1       /* one wavenumber per lane, separate z-transform kernels */
+
 /* dtype of user arrays */
 #define DPS_F32 0
 #define DPS_F64 1
@@ -57,6 +61,8 @@ extern "C" {
  *   alpha, alpha_d : kernel support half-width; <=0 selects w*h/2 (the CPU
  *                    reference's value, src/ParticleList.cpp:229).
  *   w_d/beta_d/alpha_d describe the derivative (dipole) kernel used for torques.
+ *   solver : DPS_SOLVER_* (no counterpart in the reference; both give the same
+ *            result to rounding).
  */
 typedef struct dps_params {
   double viscosity;
@@ -68,6 +74,7 @@ typedef struct dps_params {
   int nx, ny, nz;
   int mode;
   int precision;
+  int solver;
 } dps_params;
 
 typedef struct dps_handle dps_handle;

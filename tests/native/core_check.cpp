@@ -2,9 +2,12 @@
 // runs the exact per-mode BVP, stencil and table code that the gfx950 kernels
 // run, on the CPU, so tests can compare it with the FP64 oracle without a GPU.
 // Test infrastructure only (built by tests/conftest.py with g++).
+#include <barrier>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include "dps_core.h"
+#include "dps_seg.h"
 #include "dps_tables.h"
 
 using namespace dps;
@@ -30,6 +33,106 @@ struct HostOut {
     q[1] = v.im;
   }
 };
+
+// Lane exchange for the group solver: one host thread per lane, every
+// exchange is a write, a barrier, a read and a barrier.
+struct HostGroup {
+  std::barrier<> bar;
+  double buf[64];
+  explicit HostGroup(int n) : bar(n) {}
+};
+struct HostLanes {
+  int l, n;
+  HostGroup* g;
+  double exch(double v, int src) const
+  {
+    g->buf[l] = v;
+    g->bar.arrive_and_wait();
+    const double r = g->buf[src];
+    g->bar.arrive_and_wait();
+    return r;
+  }
+  double up(double v, int d) const { return exch(v, l - d >= 0 ? l - d : l); }
+  double down(double v, int d) const { return exch(v, l + d < n ? l + d : l); }
+  double xr(double v, int d) const { return exch(v, l ^ d); }
+};
+struct SegIn {
+  const double* p;   // [3][Nz][2] of this mode
+  int Nz, r0;
+  cplx operator()(int c, int q) const
+  {
+    const int row = r0 + q;
+    if (row >= Nz) return mk(0, 0);
+    const double* v = p + 2 * (c * Nz + row);
+    return mk(v[0], v[1]);
+  }
+};
+struct SegOut {
+  double* p;
+  int Nz, r0;
+  void operator()(int c, int q, cplx v)
+  {
+    double* o = p + 2 * (c * Nz + r0 + q);
+    o[0] = v.re;
+    o[1] = v.im;
+  }
+};
+
+template <int LPM, int E>
+void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, const int* nyqx,
+             const int* nyqy, const double* fin, double* uout, double* corr)
+{
+  const int Nz = B.Nz;
+  std::vector<double> ibv(Nz + 1), abv(Nz);
+  std::vector<cplx> dummy(Nz);
+  ScratchView sc;
+  sc.ib = ibv.data(); sc.ab = abv.data(); sc.stride = 1;
+  sc.P = sc.CP = sc.DP = sc.U0 = sc.U1 = sc.U2 = dummy.data();
+  for (int m = 0; m < M; ++m) {
+    const bool k0 = (kx[m] == 0 && ky[m] == 0);
+    double Gi[4] = {0, 0, 0, 0};
+    if (!k0) bvp_factor(B, kx[m] * kx[m] + ky[m] * ky[m], sc, Gi);
+    HostGroup grp(LPM);
+    ModeResult res0;
+    std::vector<std::thread> th;
+    for (int l = 0; l < LPM; ++l)
+      th.emplace_back([&, l]() {
+        HostLanes ln{l, LPM, &grp};
+        double ib[E], ab[E];
+        for (int q = 0; q < E; ++q) {
+          const int row = l * E + q;
+          ib[q] = (row < Nz && !k0) ? ibv[row + 1] : 0.0;
+          ab[q] = (row < Nz && !k0) ? abv[row] : 0.0;
+        }
+        SegIn in{fin + (size_t)m * 6 * Nz, Nz, l * E};
+        SegOut out{uout + (size_t)m * 6 * Nz, Nz, l * E};
+        ModeResult r;
+        for (int c = 0; c < 3; ++c)
+          for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
+        r.nbasis = 0;
+        if (k0) seg_bvp_k0<LPM, E>(B, ln, in, out);
+        else seg_bvp_mode<LPM, E>(B, ln, kx[m], ky[m], nyqx[m], nyqy[m], ib, ab, Gi, in, out, r);
+        if (l == 0) res0 = r;
+      });
+    for (auto& t : th) t.join();
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 4; ++b) {
+        corr[(size_t)m * 24 + (c * 4 + b) * 2] = k0 ? 0.0 : res0.corr[c][b].re;
+        corr[(size_t)m * 24 + (c * 4 + b) * 2 + 1] = k0 ? 0.0 : res0.corr[c][b].im;
+      }
+  }
+}
+
+BvpConsts host_consts(const ChebTables& T, int Nz, double H, double Hs, double eta, int geometry)
+{
+  BvpConsts B;
+  B.Nz = Nz; B.geometry = geometry; B.H = H; B.Hs = Hs; B.eta = eta;
+  B.sd0 = T.sd0.data(); B.sm2 = T.sm2.data(); B.sp2 = T.sp2.data();
+  B.S0N = T.S0N; B.S1N1 = T.S1N1;
+  B.fm1 = T.fm1.data(); B.fp1 = T.fp1.data(); B.F0N1 = T.F0N1;
+  B.R1 = T.R1.data(); B.R2 = T.R2.data(); B.L1 = T.L1.data(); B.L2 = T.L2.data();
+  return B;
+}
 }  // namespace
 
 extern "C" {
@@ -109,12 +212,7 @@ void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const 
             double* corr)
 {
   ChebTables T = build_cheb_tables(Nz, H);
-  BvpConsts B;
-  B.Nz = Nz; B.geometry = geometry; B.H = H; B.Hs = Hs; B.eta = eta;
-  B.sd0 = T.sd0.data(); B.sm2 = T.sm2.data(); B.sp2 = T.sp2.data();
-  B.S0N = T.S0N; B.S1N1 = T.S1N1;
-  B.fm1 = T.fm1.data(); B.fp1 = T.fp1.data(); B.F0N1 = T.F0N1;
-  B.R1 = T.R1.data(); B.R2 = T.R2.data(); B.L1 = T.L1.data(); B.L2 = T.L2.data();
+  BvpConsts B = host_consts(T, Nz, H, Hs, eta, geometry);
   std::vector<double> ib(Nz + 1), ab(Nz);
   std::vector<cplx> P(Nz), CP(Nz), DP(Nz), U0(Nz), U1(Nz), U2(Nz);
   ScratchView sc;
@@ -134,5 +232,22 @@ void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const 
         corr[(size_t)m * 24 + (c * 4 + b) * 2 + 1] = r.corr[c][b].im;
       }
   }
+}
+
+// The same modes through the lane-group solver with LPM lanes of E rows
+// (LPM * E >= Nz).  Returns -1 for an unsupported (LPM, E).
+int cc_bvp_seg(int Nz, double H, double Hs, double eta, int geometry, int M, const double* kx,
+               const double* ky, const int* nyqx, const int* nyqy, const double* fin, double* uout,
+               double* corr, int lpm, int e)
+{
+  ChebTables T = build_cheb_tables(Nz, H);
+  BvpConsts B = host_consts(T, Nz, H, Hs, eta, geometry);
+  if (lpm * e < Nz) return -1;
+#define SEG_CASE(LP, EE) \
+  if (lpm == LP && e == EE) { run_seg<LP, EE>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; }
+  SEG_CASE(4, 2) SEG_CASE(4, 4) SEG_CASE(8, 4) SEG_CASE(16, 2) SEG_CASE(16, 4) SEG_CASE(16, 6)
+  SEG_CASE(16, 8) SEG_CASE(32, 8)
+#undef SEG_CASE
+  return -1;
 }
 }

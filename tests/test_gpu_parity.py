@@ -30,6 +30,8 @@ CASES = [
     ("bottom_odd", "bottom", 15.5, 14.0, 9.0, 31, 28, 24, 40, True),
     ("slit_odd", "slit", 13.5, 15.5, 8.0, 27, 31, 25, 40, True),
     ("nowall_noT", "nowall", 12.0, 16.0, 9.0, 24, 32, 20, 30, False),
+    ("slit_nz96", "slit", 16.0, 16.0, 10.0, 32, 32, 96, 48, True),      # 16 lanes x 6 rows
+    ("bottom_nz150", "bottom", 12.0, 12.0, 9.0, 24, 24, 150, 30, True),  # 32 lanes x 8 rows
 ]
 
 
@@ -60,12 +62,12 @@ def oracle_for(case):
                             betaw_d=1.785 * 6 if torque else None)
 
 
-def gpu_for(case, precision):
+def gpu_for(case, precision, solver="auto"):
     from doublyperiodicstokes_amd import DPStokes, StokesParameters
     name, geom, Lx, Ly, Lz, nx, ny, nz, N, torque = case
     par = StokesParameters(viscosity=1 / (6 * np.pi), Lx=Lx, Ly=Ly, zmin=0.0, zmax=Lz, w=6,
                            w_d=6 if torque else 0, beta=1.785 * 6, beta_d=1.785 * 6,
-                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision)
+                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision, solver=solver)
     s = DPStokes()
     s.initialize(par, N)
     return s
@@ -93,6 +95,23 @@ def test_mdot_fp64_matches_oracle(case):
         ew = rel_l2(w, w_ref)
         say("  rel-L2 W = %.3e" % ew)
         assert ew <= TOL64
+    s.clear()
+
+
+@pytest.mark.parametrize("case", CASES[2:5], ids=[c[0] for c in CASES[2:5]])
+def test_lane_solver_fp64_matches_oracle(case):
+    """The one-wavenumber-per-lane spectral path (separate z-transform kernels)."""
+    say("case %s fp64 mdot, lane solver" % case[0])
+    pos, F, T = make_case(case, seed=3)
+    ref = oracle_for(case)
+    ref.set_positions(pos)
+    v_ref, w_ref = ref.mdot(F, T)
+    s = gpu_for(case, "double", solver="lane")
+    s.setPositions(pos)
+    v, w = s.Mdot(F, T)
+    ev, ew = rel_l2(v, v_ref), rel_l2(w, w_ref)
+    say("  rel-L2 U = %.3e  W = %.3e" % (ev, ew))
+    assert ev <= TOL64 and ew <= TOL64
     s.clear()
 
 
