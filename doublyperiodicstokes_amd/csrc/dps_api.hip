@@ -384,6 +384,7 @@ int allocate_fused(dps_handle* h)
   if ((rc = h->alloc(&h->d_fib, nf))) return rc;
   if ((rc = h->alloc(&h->d_fab, nf))) return rc;
   if ((rc = h->alloc(&h->d_fG, (size_t)h->Nm * 4))) return rc;
+  if ((rc = h->alloc(&h->d_corr, (size_t)h->Nm * 12))) return rc;
   double *t_ib = nullptr, *t_ab = nullptr;
   if (hipMalloc((void**)&t_ib, (size_t)(Nz + 1) * h->Nm * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&t_ab, (size_t)Nz * h->Nm * sizeof(double)) != hipSuccess) {
@@ -552,6 +553,7 @@ SpecArgs spec_args(dps_handle* h)
   A.lpm = h->sp_lpm; A.e = h->sp_e; A.G = h->sp_G;
   A.TfP = h->d_TfP; A.TiP = h->d_TiP;
   A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
+  A.cstash = h->d_corr;
   A.zpts = h->d_zpts;
   A.Hs = h->Lz;
   A.inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);

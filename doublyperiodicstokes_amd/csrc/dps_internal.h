@@ -115,6 +115,7 @@ struct SpecArgs {
   const double* fac_ib;   // [Nm][E][LPM] reciprocal pivots of each lane's rows
   const double* fac_ab;   // [Nm][E][LPM] A^{-1}B of each lane's rows
   const double* fac_G;    // [Nm][4] inverse Schur complement
+  double2* cstash;        // [Nm][12] first-solve corrections of Nyquist-touching modes
   const double* zpts;
   double Hs, inv_nxy;
 };

@@ -13,10 +13,12 @@
 //   * five-point stencils (SIMat / FIMat products): two halo rows from each
 //     neighbouring lane.
 // Lane exchange goes through a policy object L with members
-//   int l;  double up(v, d);  double down(v, d);  double xr(v, d);
-// (values from lane l-d / l+d / l^d of the group, own value when outside).
-// The gfx950 kernel binds it to __shfl_up/__shfl_down/__shfl_xor over LPM
-// lanes; the host-side checks bind it to threads joined by a barrier, so the
+//   int l;  template <int D> double up(v), down(v), xr(v);
+// up/down: the value of lane l-D / l+D of the group (any value when outside);
+// xr: the value of lane l's partner in butterfly stage D (an involution that
+// pairs the two halves of each aligned 2D-lane block, e.g. l ^ D).  The gfx950
+// kernel binds it to DPP row operations (16-lane groups) or lane permutes;
+// the host-side checks bind it to threads joined by a barrier, so the
 // identical code runs on both.
 //
 // Rows >= Nz are padding: their inputs are zero, their factors are zero, and
@@ -27,34 +29,46 @@
 
 namespace dps {
 
-template <int LPM, int N, class L>
+// Compiler-only memory fence between solver stages: row tables are re-read
+// from cache after it instead of being held in registers for the whole solve.
+#define DPS_STAGE_FENCE() __asm__ __volatile__("" ::: "memory")
+
+// Sum over the group: butterfly stages D = 1, 2, 4, ... pairing each lane with
+// a partner in the other half of its aligned 2D-lane block, so every lane ends
+// with the same total.
+template <int LPM, int N, int D = 1, class L>
 DPS_HD void seg_sum(const L& ln, double (&v)[N])
 {
-  #pragma unroll
-  for (int d = 1; d < LPM; d <<= 1)
+  if constexpr (D < LPM) {
     #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += ln.xr(v[i], d);
+    for (int i = 0; i < N; ++i) v[i] += ln.template xr<D>(v[i]);
+    seg_sum<LPM, N, 2 * D>(ln, v);
+  }
+}
+
+template <bool UP, int D, class L>
+DPS_HD double shift(const L& ln, double v)
+{
+  if constexpr (UP) return ln.template up<D>(v);
+  else return ln.template down<D>(v);
 }
 
 // Inclusive scan over the group of the affine maps y -> a + b y held by each
 // lane (UP: lane 0 applies first; !UP: lane LPM-1 applies first).  NC chains
 // with their own multipliers, NR right-hand sides per chain.
-template <int LPM, int NC, int NR, bool UP, class L>
+template <int LPM, int NC, int NR, bool UP, int D = 1, class L>
 DPS_HD void seg_scan(const L& ln, cplx (&a)[NC][NR], double (&b)[NC])
 {
-  #pragma unroll
-  for (int d = 1; d < LPM; d <<= 1) {
+  if constexpr (D < LPM) {
     double bs[NC];
     cplx as[NC][NR];
     #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      bs[c] = UP ? ln.up(b[c], d) : ln.down(b[c], d);
+      bs[c] = shift<UP, D>(ln, b[c]);
       #pragma unroll
-      for (int r = 0; r < NR; ++r)
-        as[c][r] = UP ? mk(ln.up(a[c][r].re, d), ln.up(a[c][r].im, d))
-                      : mk(ln.down(a[c][r].re, d), ln.down(a[c][r].im, d));
+      for (int r = 0; r < NR; ++r) as[c][r] = mk(shift<UP, D>(ln, a[c][r].re), shift<UP, D>(ln, a[c][r].im));
     }
-    const bool take = UP ? (ln.l >= d) : (ln.l + d < LPM);
+    const bool take = UP ? (ln.l >= D) : (ln.l + D < LPM);
     if (take)
       #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -62,6 +76,7 @@ DPS_HD void seg_scan(const L& ln, cplx (&a)[NC][NR], double (&b)[NC])
         for (int r = 0; r < NR; ++r) a[c][r] = a[c][r] + b[c] * as[c][r];
         b[c] = b[c] * bs[c];
       }
+    seg_scan<LPM, NC, NR, UP, 2 * D>(ln, a, b);
   }
 }
 
@@ -75,15 +90,38 @@ DPS_HD void seg_carry(const L& ln, const cplx (&a)[NC][NR], cplx (&cin)[NC][NR])
   for (int c = 0; c < NC; ++c)
     #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      const cplx v = UP ? mk(ln.up(a[c][r].re, 1), ln.up(a[c][r].im, 1))
-                        : mk(ln.down(a[c][r].re, 1), ln.down(a[c][r].im, 1));
+      const cplx v = mk(shift<UP, 1>(ln, a[c][r].re), shift<UP, 1>(ln, a[c][r].im));
       cin[c][r] = first ? mk(0, 0) : v;
     }
 }
 
+// Sweep multipliers of one lane's rows, formed on demand from the reciprocal
+// pivots (so they need no registers of their own):
+//   lz(q) = DM2(row-2) / pivot(row-2)   (forward elimination)
+//   bw(q) = DP2(row) / pivot(row)       (back substitution)
+template <int E>
+struct SegMul {
+  const BvpConsts* B;
+  double Ksq;
+  int Nz, r0;
+  const double (*ib)[E];
+  double ibm0, ibm1;   // reciprocal pivots of rows r0-2, r0-1 (lane below)
+  DPS_HD double lz(int q) const
+  {
+    const int row = r0 + q;
+    const double ibprev = (q >= 2) ? (*ib)[q - 2] : (q == 0 ? ibm0 : ibm1);
+    return (row >= 2 && row < Nz) ? (0.0 - B->sm2[row] * Ksq) * ibprev : 0.0;
+  }
+  DPS_HD double bw(int q) const
+  {
+    const int row = r0 + q;
+    return (row < Nz - 2) ? (0.0 - B->sp2[row] * Ksq) * (*ib)[q] : 0.0;
+  }
+};
+
 // KBPENTA forward sweep g_r = P_r - lz_r g_{r-2}, in place (LinearSolvers.h:106-120).
 template <int LPM, int E, int NR, class L>
-DPS_HD void seg_forward(const L& ln, cplx (&P)[NR][E], const double (&lz)[E])
+DPS_HD void seg_forward(const L& ln, cplx (&P)[NR][E], const SegMul<E>& mu)
 {
   cplx a[2][NR];
   double b[2] = {1.0, 1.0};
@@ -94,9 +132,10 @@ DPS_HD void seg_forward(const L& ln, cplx (&P)[NR][E], const double (&lz)[E])
   #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int p = q & 1;
+    const double lz = mu.lz(q);
     #pragma unroll
-    for (int r = 0; r < NR; ++r) a[p][r] = P[r][q] - lz[q] * a[p][r];
-    b[p] = b[p] * (-lz[q]);
+    for (int r = 0; r < NR; ++r) a[p][r] = P[r][q] - lz * a[p][r];
+    b[p] = b[p] * (-lz);
   }
   seg_scan<LPM, 2, NR, true>(ln, a, b);
   cplx y[2][NR];
@@ -104,9 +143,10 @@ DPS_HD void seg_forward(const L& ln, cplx (&P)[NR][E], const double (&lz)[E])
   #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int p = q & 1;
+    const double lz = mu.lz(q);
     #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      y[p][r] = P[r][q] - lz[q] * y[p][r];
+      y[p][r] = P[r][q] - lz * y[p][r];
       P[r][q] = y[p][r];
     }
   }
@@ -115,8 +155,9 @@ DPS_HD void seg_forward(const L& ln, cplx (&P)[NR][E], const double (&lz)[E])
 // KBPENTA backward sweep x_r = ib_r g_r - bw_r x_{r+2}, in place
 // (LinearSolvers.h:122-141).
 template <int LPM, int E, int NR, class L>
-DPS_HD void seg_backward(const L& ln, cplx (&P)[NR][E], const double (&ib)[E], const double (&bw)[E])
+DPS_HD void seg_backward(const L& ln, cplx (&P)[NR][E], const SegMul<E>& mu)
 {
+  const double (&ib)[E] = *mu.ib;
   cplx a[2][NR];
   double b[2] = {1.0, 1.0};
   #pragma unroll
@@ -126,9 +167,10 @@ DPS_HD void seg_backward(const L& ln, cplx (&P)[NR][E], const double (&ib)[E], c
   #pragma unroll
   for (int q = E - 1; q >= 0; --q) {
     const int p = q & 1;
+    const double bw = mu.bw(q);
     #pragma unroll
-    for (int r = 0; r < NR; ++r) a[p][r] = ib[q] * P[r][q] - bw[q] * a[p][r];
-    b[p] = b[p] * (-bw[q]);
+    for (int r = 0; r < NR; ++r) a[p][r] = ib[q] * P[r][q] - bw * a[p][r];
+    b[p] = b[p] * (-bw);
   }
   seg_scan<LPM, 2, NR, false>(ln, a, b);
   cplx y[2][NR];
@@ -136,9 +178,10 @@ DPS_HD void seg_backward(const L& ln, cplx (&P)[NR][E], const double (&ib)[E], c
   #pragma unroll
   for (int q = E - 1; q >= 0; --q) {
     const int p = q & 1;
+    const double bw = mu.bw(q);
     #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      y[p][r] = ib[q] * P[r][q] - bw[q] * y[p][r];
+      y[p][r] = ib[q] * P[r][q] - bw * y[p][r];
       P[r][q] = y[p][r];
     }
   }
@@ -152,8 +195,8 @@ DPS_HD void seg_halo(const L& ln, const cplx (&X)[E], int Nz, cplx Y0, cplx Y1, 
 {
   #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const cplx u = mk(ln.up(X[E - 2 + i].re, 1), ln.up(X[E - 2 + i].im, 1));
-    const cplx v = mk(ln.down(X[i].re, 1), ln.down(X[i].im, 1));
+    const cplx u = mk(ln.template up<1>(X[E - 2 + i].re), ln.template up<1>(X[E - 2 + i].im));
+    const cplx v = mk(ln.template down<1>(X[i].re), ln.template down<1>(X[i].im));
     lo[i] = (ln.l == 0) ? mk(0, 0) : u;
     if (ln.l == LPM - 1) {
       const int row = LPM * E + i;
@@ -163,6 +206,11 @@ DPS_HD void seg_halo(const L& ln, const cplx (&X)[E], int Nz, cplx Y0, cplx Y1, 
     }
   }
 }
+
+// Velocity component solved at position i: one at a time the z component goes
+// first, so the pressure derivative Dp is dead before the x and y solves.
+template <int NRV>
+DPS_HD int vel_comp(int i) { return NRV == 1 ? (i + 2) % 3 : i; }
 
 // S-band product of DPTools.cpp:266-284 at one row (same association order as
 // the sequential solver).
@@ -177,7 +225,6 @@ DPS_HD cplx s_band(const BvpConsts& B, int row, cplx xm2, cplx x0, cplx x2, cplx
 }
 
 // Rows of the Schur solution x - A^{-1}B Y, extended by Y0, Y1 at rows Nz, Nz+1.
-template <int E>
 DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y0, cplx Y1)
 {
   if (row < Nz) return x - ((row & 1) ? ab * Y1 : ab * Y0);
@@ -186,13 +233,14 @@ DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y0, cplx Y1)
   return mk(0, 0);
 }
 
-// One mode k != 0 (counterpart of bvp_mode).  ib/ab: this lane's rows of the
+// One mode k != 0 (counterpart of bvp_mode); NRV (1 or 3) velocity components
+// share each scan.  ib/ab: this lane's rows of the
 // static factorisation (ib = 1/pivot of the row, i.e. IB(row+1)); Gi: inverse
 // Schur complement.  fin(c, q) returns the forcing coefficient of component c
 // at row l*E+q (zero for rows >= Nz); fout(c, q, v) receives the velocity
 // coefficients of the rows < Nz; res gets the wall correction (identical on
 // every lane of the group).
-template <int LPM, int E, class L, class FIn, class FOut>
+template <int LPM, int E, int NRV = 3, class L, class FIn, class FOut>
 DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, int nyq_x, int nyq_y,
                          const double (&ib)[E], const double (&ab)[E], const double* Gi, const FIn& fin,
                          FOut& fout, ModeResult& res)
@@ -205,21 +253,19 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
   const cplx Dy = nyq_y ? mk(0, 0) : mk(0, ky);
   const int r0 = ln.l * E;
+  const double iH = 1.0 / H;
 #define C0(j) (B.R2[(j)] * H2K + (B.R2[(j)] + (H * B.R1[(j)] - B.R2[(j)])))
 #define C1(j) (B.L2[(j)] * H2K + (-B.L2[(j)] + (H * B.L1[(j)] + B.L2[(j)])))
 
   // sweep multipliers of this lane's rows
-  double lz[E], bw[E];
-  {
-    const double ibm0 = ln.up(ib[E - 2], 1), ibm1 = ln.up(ib[E - 1], 1);
-    #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int row = r0 + q;
-      const double ibprev = (q >= 2) ? ib[q - 2] : (q == 0 ? ibm0 : ibm1);
-      lz[q] = (row >= 2 && row < Nz) ? (0.0 - B.sm2[row] * Ksq) * ibprev : 0.0;
-      bw[q] = (row < Nz - 2) ? (0.0 - B.sp2[row] * Ksq) * ib[q] : 0.0;
-    }
-  }
+  SegMul<E> mu;
+  mu.B = &B;
+  mu.Ksq = Ksq;
+  mu.Nz = Nz;
+  mu.r0 = r0;
+  mu.ib = &ib;
+  mu.ibm0 = ln.template up<1>(ib[E - 2]);
+  mu.ibm1 = ln.template up<1>(ib[E - 1]);
 
   // ---- pressure rhs: Dx fx + Dy fy + chebCoeffDiff(fz) (DPTools.cpp:134-195) ----
   cplx X[1][E];
@@ -229,7 +275,7 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     #pragma unroll
     for (int q = E - 1; q >= 0; --q) {
       const int row = r0 + q;
-      const cplx w = (row < Nz) ? ((double)(row * 2) / H) * fin(2, q) : mk(0, 0);
+      const cplx w = (row < Nz) ? ((double)(row * 2) * iH) * fin(2, q) : mk(0, 0);
       s[q & 1][0] = s[q & 1][0] + w;
       suf[q] = s[q & 1][0];
     }
@@ -245,8 +291,11 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
       X[0][q] = (row < Nz) ? Dx * fin(0, q) + Dy * fin(1, q) + df : mk(0, 0);
     }
   }
-  seg_forward<LPM, E, 1>(ln, X, lz);
-  seg_backward<LPM, E, 1>(ln, X, ib, bw);
+  DPS_STAGE_FENCE();
+  seg_forward<LPM, E, 1>(ln, X, mu);
+  DPS_STAGE_FENCE();
+  seg_backward<LPM, E, 1>(ln, X, mu);
+  DPS_STAGE_FENCE();
   cplx Y0, Y1;
   {
     double y[4] = {0, 0, 0, 0};
@@ -266,7 +315,7 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   }
   // x = xp - A^{-1}B Y ; Cp = SIMat x ; Dp = FIMat x  (DPTools.cpp:266-284)
   #pragma unroll
-  for (int q = 0; q < E; ++q) X[0][q] = seg_xrow<E>(r0 + q, Nz, X[0][q], ab[q], Y0, Y1);
+  for (int q = 0; q < E; ++q) X[0][q] = seg_xrow(r0 + q, Nz, X[0][q], ab[q], Y0, Y1);
   cplx Cp[E], Dp[E];
   cplx psum, palt;
   {
@@ -298,80 +347,96 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     psum = mk(red[0], red[1]);
     palt = mk(red[2], red[3]);
   }
-  // BC rows for the velocity solves (DPTools.cpp:218-234)
-  const double fac1 = 2 * B.eta;
-  const double facK = fac1 * K;
-  cplx bc[6];
-  bc[0] = ((-1.0) * Dx) * psum / facK;
-  bc[1] = Dx * palt / facK;
-  bc[2] = ((-1.0) * Dy) * psum / facK;
-  bc[3] = Dy * palt / facK;
-  bc[4] = psum / fac1;
-  bc[5] = palt / fac1;
+  // BC rows for the velocity solves (DPTools.cpp:218-234), formed where used:
+  //   bc[2c] = -D_c psum / (2 eta K), bc[2c+1] = D_c palt / (2 eta K)  (c = 0, 1)
+  //   bc[4] = psum / (2 eta),         bc[5] = palt / (2 eta)
+  const double ifac1 = 1.0 / (2 * B.eta);
+  const double ifacK = 1.0 / ((2 * B.eta) * K);
 
-  // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345) ----
-  cplx U[3][E];
+  // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345), in
+  //      batches of NRV components sharing each scan ----
+  const double ieta = 1.0 / B.eta;
+  cplx wall_bot[3], wall_top[3];
   #pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int row = r0 + q;
-    if (row < Nz) {
-      U[0][q] = (Dx * Cp[q] - fin(0, q)) / B.eta;
-      U[1][q] = (Dy * Cp[q] - fin(1, q)) / B.eta;
-      U[2][q] = (Dp[q] - fin(2, q)) / B.eta;
-    } else {
-      U[0][q] = U[1][q] = U[2][q] = mk(0, 0);
-    }
-  }
-  seg_forward<LPM, E, 3>(ln, U, lz);
-  seg_backward<LPM, E, 3>(ln, U, ib, bw);
-  cplx Ya[3], Yb[3];
-  {
-    double y[12];
-    #pragma unroll
-    for (int i = 0; i < 12; ++i) y[i] = 0;
+  for (int cb = 0; cb < 3; cb += NRV) {
+    DPS_STAGE_FENCE();
+    cplx U[NRV][E];
     #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int row = r0 + q;
-      if (row < Nz) {
-        const double c0 = C0(row), c1 = C1(row);
-        #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          y[4 * c + 0] += U[c][q].re * c0; y[4 * c + 1] += U[c][q].im * c0;
-          y[4 * c + 2] += U[c][q].re * c1; y[4 * c + 3] += U[c][q].im * c1;
+      #pragma unroll
+      for (int b = 0; b < NRV; ++b) {
+        const int c = vel_comp<NRV>(cb + b);
+        if (row < Nz) {
+          const cplx g = (c == 0) ? Dx * Cp[q] : ((c == 1) ? Dy * Cp[q] : Dp[q]);
+          U[b][q] = (g - fin(c, q)) * ieta;
+        } else {
+          U[b][q] = mk(0, 0);
         }
       }
     }
-    seg_sum<LPM>(ln, y);
-    #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const cplx a = mk(y[4 * c], y[4 * c + 1]) - bc[2 * c];
-      const cplx b2 = mk(y[4 * c + 2], y[4 * c + 3]) - bc[2 * c + 1];
-      Ya[c] = Gi[0] * a + Gi[1] * b2;
-      Yb[c] = Gi[2] * a + Gi[3] * b2;
-    }
-  }
-  cplx wall_bot[3], wall_top[3];
-  {
-    double red[12];
-    #pragma unroll
-    for (int i = 0; i < 12; ++i) red[i] = 0;
-    #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    DPS_STAGE_FENCE();
+    seg_forward<LPM, E, NRV>(ln, U, mu);
+    DPS_STAGE_FENCE();
+    seg_backward<LPM, E, NRV>(ln, U, mu);
+    DPS_STAGE_FENCE();
+    cplx Ya[NRV], Yb[NRV];
+    {
+      double y[4 * NRV];
       #pragma unroll
-      for (int q = 0; q < E; ++q) U[c][q] = seg_xrow<E>(r0 + q, Nz, U[c][q], ab[q], Ya[c], Yb[c]);
+      for (int i = 0; i < 4 * NRV; ++i) y[i] = 0;
+      #pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int row = r0 + q;
+        if (row < Nz) {
+          const double c0 = C0(row), c1 = C1(row);
+          #pragma unroll
+          for (int b = 0; b < NRV; ++b) {
+            y[4 * b + 0] += U[b][q].re * c0; y[4 * b + 1] += U[b][q].im * c0;
+            y[4 * b + 2] += U[b][q].re * c1; y[4 * b + 3] += U[b][q].im * c1;
+          }
+        }
+      }
+      seg_sum<LPM>(ln, y);
+      #pragma unroll
+      for (int b = 0; b < NRV; ++b) {
+        const int c = vel_comp<NRV>(cb + b);
+        cplx bc0, bc1;
+        if (c == 2) {
+          bc0 = psum * ifac1;
+          bc1 = palt * ifac1;
+        } else {
+          const cplx Dc = (c == 0) ? Dx : Dy;
+          bc0 = ((-1.0) * Dc) * psum * ifacK;
+          bc1 = Dc * palt * ifacK;
+        }
+        const cplx a = mk(y[4 * b], y[4 * b + 1]) - bc0;
+        const cplx b2 = mk(y[4 * b + 2], y[4 * b + 3]) - bc1;
+        Ya[b] = Gi[0] * a + Gi[1] * b2;
+        Yb[b] = Gi[2] * a + Gi[3] * b2;
+      }
+    }
+    double red[4 * NRV];
+    #pragma unroll
+    for (int i = 0; i < 4 * NRV; ++i) red[i] = 0;
+    #pragma unroll
+    for (int b = 0; b < NRV; ++b) {
+      const int c = vel_comp<NRV>(cb + b);
+      #pragma unroll
+      for (int q = 0; q < E; ++q) U[b][q] = seg_xrow(r0 + q, Nz, U[b][q], ab[q], Ya[b], Yb[b]);
       cplx lo[2], hi[2];
-      seg_halo<LPM, E>(ln, U[c], Nz, Ya[c], Yb[c], lo, hi);
+      seg_halo<LPM, E>(ln, U[b], Nz, Ya[b], Yb[b], lo, hi);
       cplx cu[E];
       #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int row = r0 + q;
         if (row < Nz) {
-          const cplx xm2 = (q >= 2) ? U[c][q - 2] : lo[q];
-          const cplx x2 = (q + 2 < E) ? U[c][q + 2] : hi[q + 2 - E];
-          cu[q] = s_band(B, row, xm2, U[c][q], x2, Ya[c], Yb[c]);
-          red[4 * c + 0] += cu[q].re; red[4 * c + 1] += cu[q].im;
+          const cplx xm2 = (q >= 2) ? U[b][q - 2] : lo[q];
+          const cplx x2 = (q + 2 < E) ? U[b][q + 2] : hi[q + 2 - E];
+          cu[q] = s_band(B, row, xm2, U[b][q], x2, Ya[b], Yb[b]);
+          red[4 * b + 0] += cu[q].re; red[4 * b + 1] += cu[q].im;
           const double sg = (row & 1) ? -1.0 : 1.0;
-          red[4 * c + 2] += sg * cu[q].re; red[4 * c + 3] += sg * cu[q].im;
+          red[4 * b + 2] += sg * cu[q].re; red[4 * b + 3] += sg * cu[q].im;
         }
       }
       #pragma unroll
@@ -380,9 +445,9 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     }
     seg_sum<LPM>(ln, red);
     #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      wall_top[c] = mk(red[4 * c], red[4 * c + 1]);
-      wall_bot[c] = mk(red[4 * c + 2], red[4 * c + 3]);
+    for (int b = 0; b < NRV; ++b) {
+      wall_top[vel_comp<NRV>(cb + b)] = mk(red[4 * b], red[4 * b + 1]);
+      wall_bot[vel_comp<NRV>(cb + b)] = mk(red[4 * b + 2], red[4 * b + 3]);
     }
   }
 #undef C0
@@ -406,6 +471,7 @@ DPS_HD void seg_bvp_k0(const BvpConsts& B, const L& ln, const FIn& fin, FOut& fo
   }
   const double* RA = (B.geometry == MODE_BOTTOM) ? B.R1 : B.R2;
   const double* RB = B.L2;
+  const double ieta = 1.0 / B.eta;
   const double D00 = -RA[Nz], D01 = -RA[Nz + 1], D10 = -RB[Nz], D11 = -RB[Nz + 1];
   const double idet = 1.0 / (D00 * D11 - D01 * D10);
   const double g00 = idet * D11, g01 = idet * (-D01), g10 = idet * (-D10), g11 = idet * D00;
@@ -419,7 +485,7 @@ DPS_HD void seg_bvp_k0(const BvpConsts& B, const L& ln, const FIn& fin, FOut& fo
       if (row < Nz) {
         cplx f = fin(c, q);
         if (B.geometry == MODE_SLIT) f = mk(f.re, f.re);   // Solvers.py:564-566
-        X[q] = (-1.0) * f / B.eta;
+        X[q] = f * (-ieta);
         red[0] += RA[row] * X[q].re; red[1] += RA[row] * X[q].im;
         red[2] += RB[row] * X[q].re; red[3] += RB[row] * X[q].im;
       } else {

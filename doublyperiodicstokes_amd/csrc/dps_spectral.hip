@@ -9,29 +9,70 @@
 //   phase 0  the G columns of the r2c spectrum, all z levels and components,
 //            are read once from HBM (rows of G consecutive modes) into LDS
 //   phase 1  forward transform: lane l forms the coefficients of its rows
-//            from even/odd folded node pairs (grid precision)
-//   phase 2  FP64 BVP on the lane group; modes touching a Nyquist index are
-//            solved twice (mode and mirror) and averaged in LDS
+//            from even/odd folded node pairs (grid precision), written back
+//            in place
+//   phase 2  FP64 BVP on the lane group, in place; modes touching a Nyquist
+//            index are solved twice (mode and mirror) and averaged, the
+//            mirror reading a copy of the forcing kept in its output column
 //   phase 3  inverse transform of node pairs (l + LPM r, Nz-1-l-LPM r) plus
-//            the analytic wall profiles, staged in LDS
+//            the analytic wall profiles, written back in place
 //   phase 4  the G columns are written back as rows of G consecutive modes
 //
 // The spectrum crosses HBM once in each direction; everything in between
-// stays in LDS and registers.
+// stays in one LDS buffer of 3 Nz G elements and registers.
 #include <hip/hip_runtime.h>
 #include "dps_internal.h"
 #include "dps_seg.h"
+
+// velocity components sharing each scan of the lane-group solve
+#define DPS_SPEC_NRV 1
 
 namespace dps {
 
 namespace {
 
+// 64-bit DPP move within a 16-lane row; lanes whose source is outside the row
+// keep their own value
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v)
+{
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane exchange of one mode's group.  A 16-lane group is exactly one DPP row:
+// shifts are row_shr/row_shl and the butterfly partners are quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror and row_mirror, all VALU
+// operations.  Wider groups use lane permutes.
 template <int LPM>
 struct DevLanes {
   int l;
-  __device__ double up(double v, int d) const { return __shfl_up(v, (unsigned)d, LPM); }
-  __device__ double down(double v, int d) const { return __shfl_down(v, (unsigned)d, LPM); }
-  __device__ double xr(double v, int d) const { return __shfl_xor(v, d, LPM); }
+  template <int D>
+  __device__ double up(double v) const
+  {
+    if constexpr (LPM == 16) return dpp64<0x110 + D>(v);   // row_shr:D
+    else return __shfl_up(v, (unsigned)D, LPM);
+  }
+  template <int D>
+  __device__ double down(double v) const
+  {
+    if constexpr (LPM == 16) return dpp64<0x100 + D>(v);   // row_shl:D
+    else return __shfl_down(v, (unsigned)D, LPM);
+  }
+  template <int D>
+  __device__ double xr(double v) const
+  {
+    if constexpr (LPM == 16) {
+      if constexpr (D == 1) return dpp64<0xB1>(v);         // quad_perm [1,0,3,2]
+      else if constexpr (D == 2) return dpp64<0x4E>(v);    // quad_perm [2,3,0,1]
+      else if constexpr (D == 4) return dpp64<0x141>(v);   // row_half_mirror
+      else return dpp64<0x140>(v);                         // row_mirror
+    } else {
+      return __shfl_xor(v, D, LPM);
+    }
+  }
 };
 
 // coefficient rows of one mode in LDS, layout [c][row][g]
@@ -69,7 +110,32 @@ struct LdsStore {
   }
 };
 
-template <class T, int LPM, int E>
+// Global-memory view of the coefficients of one mode (the stash that keeps the
+// forcing coefficients of a Nyquist-touching mode for its mirror solve).
+template <class T>
+struct GlobalCoef {
+  const typename C2<T>::t* p;   // element (c, row) at p[(c * Nz + row) * Nm]
+  long Nm;
+  int Nz, r0;
+  __device__ cplx operator()(int c, int q) const
+  {
+    const int row = r0 + q;
+    if (row >= Nz) return mk(0, 0);
+    const typename C2<T>::t v = p[(size_t)(c * Nz + row) * Nm];
+    return mk((double)v.x, -(double)v.y);   // conjugated input of the mirror
+  }
+};
+
+// Block index -> mode group: consecutive groups run on the same XCD (blocks
+// are dispatched to the 8 XCDs round robin), so the two halves of a 128-byte
+// line shared by neighbouring groups meet in one L2.
+__device__ __forceinline__ int xcd_group(int b, int nb)
+{
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+template <class T, int LPM, int E, int NRV>
 __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<T>::t* __restrict__ in,
                                                   typename C2<T>::t* __restrict__ out)
 {
@@ -78,13 +144,20 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   extern __shared__ unsigned char smem_raw[];
   const int Nz = A.B.Nz, G = A.G, Nm = A.Nm, NH = A.NH;
   const int ncol = 3 * Nz * G;
-  V* S0 = reinterpret_cast<V*>(smem_raw);   // values, then velocity coefficients
-  V* S1 = S0 + ncol;                        // forcing coefficients, then values
-  const int m0 = blockIdx.x * G;
+  V* S = reinterpret_cast<V*>(smem_raw);   // [c][row][g]: values, coefficients, values
+  const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
   const int g = threadIdx.x / LPM;
   const int l = threadIdx.x % LPM;
   const int m = m0 + g;
   const int r0 = l * E;
+  int ix = 0, iy = 0, nyqx = 0, nyqy = 0;
+  if (m < Nm) {
+    iy = m / A.Nkx;
+    ix = m - iy * A.Nkx;
+    nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
+    nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+  }
+  const bool special = (m > 0) && (m < Nm) && (nyqx || nyqy);
 
   // ---- phase 0: columns in ----
   for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
@@ -92,7 +165,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     V v;
     v.x = v.y = (T)0;
     if (m0 + gi < Nm) v = in[(size_t)row * Nm + m0 + gi];
-    S0[i] = v;
+    S[i] = v;
   }
   __syncthreads();
 
@@ -104,13 +177,14 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
 #pragma unroll
       for (int q = 0; q < E; ++q) ar[c][q] = ai[c][q] = (T)0;
     const T* Tf = (const T*)A.TfP;
+#pragma unroll 2
     for (int j = 0; j < NH; ++j) {
       T er[3], ei[3], orr[3], oi[3];
       const bool mid = (j == Nz - 1 - j);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const V a = S0[(c * Nz + j) * G + g];
-        const V b = S0[(c * Nz + Nz - 1 - j) * G + g];
+        const V a = S[(c * Nz + j) * G + g];
+        const V b = S[(c * Nz + Nz - 1 - j) * G + g];
         if (mid) { er[c] = a.x; ei[c] = a.y; orr[c] = oi[c] = (T)0; }
         else { er[c] = a.x + b.x; ei[c] = a.y + b.y; orr[c] = a.x - b.x; oi[c] = a.y - b.y; }
       }
@@ -125,6 +199,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
           ai[c][q] += t[q] * ((q & 1) ? oi[c] : ei[c]);
         }
     }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q)
       if (r0 + q < Nz)
@@ -133,12 +208,12 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
           V v;
           v.x = ar[c][q];
           v.y = ai[c][q];
-          S1[(c * Nz + r0 + q) * G + g] = v;
+          S[(c * Nz + r0 + q) * G + g] = v;
+          if (special) out[(size_t)(c * Nz + r0 + q) * Nm + m] = v;
         }
   }
-  __syncthreads();
 
-  // ---- phase 2: FP64 BVP of mode m on this lane group ----
+  // ---- phase 2: FP64 BVP of mode m on this lane group, in place ----
   ModeResult res;
   res.nbasis = 0;
 #pragma unroll
@@ -148,16 +223,13 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   double K = 0;
   if (m < Nm) {
     const DevLanes<LPM> ln{l};
-    LdsCoef<T> fin{S1, G, g, Nz, r0, false};
-    LdsStore<T> fout{S0, G, g, Nz, r0, false};
+    LdsCoef<T> fin{S, G, g, Nz, r0, false};
+    LdsStore<T> fout{S, G, g, Nz, r0, false};
     if (m == 0) {
       seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);
     } else {
-      const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
       const double kx = kvec(ix, A.Nx, A.Lx), ky = kvec(iy, A.Ny, A.Ly);
       K = sqrt(kx * kx + ky * ky);
-      const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
-      const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
       double ib[E], ab[E], Gi[4];
 #pragma unroll
       for (int q = 0; q < E; ++q) {
@@ -166,25 +238,34 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) Gi[i] = A.fac_G[(size_t)m * 4 + i];
-      seg_bvp_mode<LPM, E>(A.B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res);
-      if (nyqx || nyqy) {
+      seg_bvp_mode<LPM, E, NRV>(A.B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res);
+      if (special) {
         // the reference's c2c transform keeps only the real part of the
         // inverse: the Hermitian average of this mode and its mirror, solved
-        // with the mirror's own grid wavenumbers (same |k|, same factors)
+        // with the mirror's own grid wavenumbers (same |k|, same factors) on
+        // the conjugated forcing kept in the output column
         const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
         const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
         const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
         const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
-        fin.conj = true;
-        fout.avg = true;
-        ModeResult r2;
-        seg_bvp_mode<LPM, E>(A.B, ln, kxm, kym, nyqxm, nyqym, ib, ab, Gi, fin, fout, r2);
+        // the first solve's correction waits in memory (every lane of the
+        // group writes the same values), not in registers
+        double2* cs = A.cstash + (size_t)m * 12;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int b = 0; b < 4; ++b)
-            res.corr[c][b] = mk(0.5 * (res.corr[c][b].re + r2.corr[c][b].re),
-                                0.5 * (res.corr[c][b].im - r2.corr[c][b].im));
+          for (int b = 0; b < 4; ++b) cs[c * 4 + b] = make_double2(res.corr[c][b].re, res.corr[c][b].im);
+        DPS_STAGE_FENCE();
+        GlobalCoef<T> fin2{out + m, (long)Nm, Nz, r0};
+        fout.avg = true;
+        seg_bvp_mode<LPM, E, NRV>(A.B, ln, kxm, kym, nyqxm, nyqym, ib, ab, Gi, fin2, fout, res);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const double2 a = cs[c * 4 + b];
+            res.corr[c][b] = mk(0.5 * (a.x + res.corr[c][b].re), 0.5 * (a.y - res.corr[c][b].im));
+          }
       }
     }
   }
@@ -198,10 +279,11 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < R; ++r) er[c][r] = ei[c][r] = orr[c][r] = oi[c][r] = (T)0;
+#pragma unroll 2
     for (int k = 0; k < Nz; k += 2) {
       V u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S0[(c * Nz + k) * G + g];
+      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * G + g];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const T t = Ti[((size_t)k * R + r) * LPM + l];
@@ -212,10 +294,11 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
         }
       }
     }
+#pragma unroll 2
     for (int k = 1; k < Nz; k += 2) {
       V u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S0[(c * Nz + k) * G + g];
+      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * G + g];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const T t = Ti[((size_t)k * R + r) * LPM + l];
@@ -226,21 +309,32 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
         }
       }
     }
+    __syncthreads();
     const int nb = res.nbasis;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = l + LPM * r;
       if (j >= NH) continue;
+      const int jm = Nz - 1 - j;
+      // z_{Nz-1-j} = Lz - z_j, so e^{K(z - Lz)} at one node of the pair is
+      // e^{-K z} at the other
+      double ej = 0, em = 0;
+      if (nb) {
+        ej = exp(-K * A.zpts[j]);
+        em = (jm == j) ? ej : exp(-K * A.zpts[jm]);
+      }
       for (int side = 0; side < 2; ++side) {
-        const int jj = side ? Nz - 1 - j : j;
+        const int jj = side ? jm : j;
         if (side && jj == j) break;
-        double pr[3] = {0, 0, 0}, pi[3] = {0, 0, 0};
-        if (nb) {
-          const double z = A.zpts[jj];
-          const double e1 = exp(-K * z);
-          const double e2 = (nb == 4) ? exp(K * (z - A.Hs)) : 0.0;
+        const double z = A.zpts[jj];
+        const double e1 = side ? em : ej;
+        const double e2 = side ? ej : em;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < 3; ++c) {
+          V v;
+          v.x = side ? (er[c][r] - orr[c][r]) : (er[c][r] + orr[c][r]);
+          v.y = side ? (ei[c][r] - oi[c][r]) : (ei[c][r] + oi[c][r]);
+          if (nb) {
             double sr = res.corr[c][0].re * e1, si = res.corr[c][0].im * e1;
             if (nb == 2) {
               sr += res.corr[c][1].re * (z * e1);
@@ -249,20 +343,10 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
               sr += res.corr[c][1].re * e2 + res.corr[c][2].re * (z * e1) + res.corr[c][3].re * (z * e2);
               si += res.corr[c][1].im * e2 + res.corr[c][2].im * (z * e1) + res.corr[c][3].im * (z * e2);
             }
-            pr[c] = sr * A.inv_nxy;
-            pi[c] = si * A.inv_nxy;
+            v.x += (T)(sr * A.inv_nxy);
+            v.y += (T)(si * A.inv_nxy);
           }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          V v;
-          v.x = side ? (er[c][r] - orr[c][r]) : (er[c][r] + orr[c][r]);
-          v.y = side ? (ei[c][r] - oi[c][r]) : (ei[c][r] + oi[c][r]);
-          if (nb) {
-            v.x += (T)pr[c];
-            v.y += (T)pi[c];
-          }
-          S1[(c * Nz + jj) * G + g] = v;
+          S[(c * Nz + jj) * G + g] = v;
         }
       }
     }
@@ -272,7 +356,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   // ---- phase 4: columns out ----
   for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
     const int gi = i % G, row = i / G;
-    if (m0 + gi < Nm) out[(size_t)row * Nm + m0 + gi] = S1[i];
+    if (m0 + gi < Nm) out[(size_t)row * Nm + m0 + gi] = S[i];
   }
 }
 
@@ -319,7 +403,7 @@ bool spectral_shape(int Nz, int* lpm, int* e)
 
 int spectral_modes_per_block(int Nz, int lpm, bool mixed)
 {
-  const size_t per_mode = (size_t)2 * 3 * Nz * (mixed ? sizeof(float2) : sizeof(double2));
+  const size_t per_mode = (size_t)3 * Nz * (mixed ? sizeof(float2) : sizeof(double2));
   int G = 256 / lpm;
   while (G > 1 && (size_t)G * per_mode > 65536) G /= 2;
   return G;
@@ -337,13 +421,13 @@ template <class T>
 void launch_spectral(const SpecArgs& A, const void* in, void* out, hipStream_t s)
 {
   typedef typename C2<T>::t V;
-  const size_t lds = (size_t)2 * 3 * A.B.Nz * A.G * sizeof(V);
+  const size_t lds = (size_t)3 * A.B.Nz * A.G * sizeof(V);
   const dim3 grid((A.Nm + A.G - 1) / A.G), block(A.G * A.lpm);
   const V* i = (const V*)in;
   V* o = (V*)out;
 #define SPEC_CASE(LP, EE)                                                                    \
   if (A.lpm == LP && A.e == EE) {                                                            \
-    hipLaunchKernelGGL((k_spectral<T, LP, EE>), grid, block, lds, s, A, i, o);               \
+    hipLaunchKernelGGL((k_spectral<T, LP, EE, DPS_SPEC_NRV>), grid, block, lds, s, A, i, o); \
     return;                                                                                  \
   }
   SPEC_CASE(16, 2) SPEC_CASE(16, 4) SPEC_CASE(16, 6) SPEC_CASE(16, 8) SPEC_CASE(32, 8) SPEC_CASE(64, 8)

@@ -52,9 +52,9 @@ struct HostLanes {
     g->bar.arrive_and_wait();
     return r;
   }
-  double up(double v, int d) const { return exch(v, l - d >= 0 ? l - d : l); }
-  double down(double v, int d) const { return exch(v, l + d < n ? l + d : l); }
-  double xr(double v, int d) const { return exch(v, l ^ d); }
+  template <int D> double up(double v) const { return exch(v, l - D >= 0 ? l - D : l); }
+  template <int D> double down(double v) const { return exch(v, l + D < n ? l + D : l); }
+  template <int D> double xr(double v) const { return exch(v, l ^ D); }
 };
 struct SegIn {
   const double* p;   // [3][Nz][2] of this mode
@@ -78,7 +78,7 @@ struct SegOut {
   }
 };
 
-template <int LPM, int E>
+template <int LPM, int E, int NRV>
 void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, const int* nyqx,
              const int* nyqy, const double* fin, double* uout, double* corr)
 {
@@ -111,7 +111,7 @@ void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, cons
           for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
         r.nbasis = 0;
         if (k0) seg_bvp_k0<LPM, E>(B, ln, in, out);
-        else seg_bvp_mode<LPM, E>(B, ln, kx[m], ky[m], nyqx[m], nyqy[m], ib, ab, Gi, in, out, r);
+        else seg_bvp_mode<LPM, E, NRV>(B, ln, kx[m], ky[m], nyqx[m], nyqy[m], ib, ab, Gi, in, out, r);
         if (l == 0) res0 = r;
       });
     for (auto& t : th) t.join();
@@ -235,16 +235,18 @@ void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const 
 }
 
 // The same modes through the lane-group solver with LPM lanes of E rows
-// (LPM * E >= Nz).  Returns -1 for an unsupported (LPM, E).
+// (LPM * E >= Nz), nrv velocity components per scan.  Returns -1 for an
+// unsupported shape.
 int cc_bvp_seg(int Nz, double H, double Hs, double eta, int geometry, int M, const double* kx,
                const double* ky, const int* nyqx, const int* nyqy, const double* fin, double* uout,
-               double* corr, int lpm, int e)
+               double* corr, int lpm, int e, int nrv)
 {
   ChebTables T = build_cheb_tables(Nz, H);
   BvpConsts B = host_consts(T, Nz, H, Hs, eta, geometry);
   if (lpm * e < Nz) return -1;
-#define SEG_CASE(LP, EE) \
-  if (lpm == LP && e == EE) { run_seg<LP, EE>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; }
+#define SEG_CASE(LP, EE)                                                                     \
+  if (lpm == LP && e == EE && nrv == 3) { run_seg<LP, EE, 3>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; } \
+  if (lpm == LP && e == EE && nrv == 1) { run_seg<LP, EE, 1>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; }
   SEG_CASE(4, 2) SEG_CASE(4, 4) SEG_CASE(8, 4) SEG_CASE(16, 2) SEG_CASE(16, 4) SEG_CASE(16, 6)
   SEG_CASE(16, 8) SEG_CASE(32, 8)
 #undef SEG_CASE
