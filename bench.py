@@ -23,6 +23,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+HERE = os.path.dirname(os.path.abspath(__file__))
+# kernel that runs each stage (the "bvp" stage is the fused spectral kernel)
+STAGE_KERNEL = {"spread": "k_spread", "fft_fwd": "rocfft", "z_fwd": "k_spectral", "bvp": "k_spectral",
+                "z_inv": "k_spectral", "fft_inv": "rocfft", "interp": "k_interp"}
 
 WORKLOADS = {
     # name: (geometry, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque)
@@ -81,6 +85,33 @@ def time_steps(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) * 1000.0 / steps
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading for a copy
+    kernel of known size), or None when no summary covers this kernel."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    e = d.get(workload, {}).get(kernel)
+    if not e:
+        return None, None
+    return e["bytes_per_launch"], e["source"]
+
+
+def copy_probe(torch, mb):
+    """A plain device copy of known size (reads and writes `mb` MiB), run so
+    that a profiler can calibrate its byte counters against it."""
+    n = mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def cpu_baseline(wl, pos, F, T, budget_s):
     """FP64 oracle Mdot timed on the host: as many full calls as fit in the budget (at least one)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -111,6 +142,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU-baseline Mdot work (0 disables)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain copy of this many MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -140,9 +173,13 @@ def main():
         acc = t if acc is None else {k: acc[k] + t[k] for k in acc}
     mixed.set_profiling(False)
     stage_us = {k: v / nprof for k, v in acc.items() if k != "locate"}
+    if stage_us["z_fwd"] + stage_us["z_inv"] < 0.05 * stage_us["bvp"]:
+        # fused spectral kernel: the z transforms run inside the "bvp" launch
+        del stage_us["z_fwd"], stage_us["z_inv"]
     sb = stage_bytes(wl, 4)
     stage_gbs = {k: sb[k] / (stage_us[k] * 1e-6) / 1e9 for k in stage_us}
     dom = max(stage_us, key=lambda k: stage_us[k])
+    traffic, traffic_src = measured_traffic(args.workload, STAGE_KERNEL[dom])
 
     out = {
         "metric": "Mdot solves/sec, N=16k slit, mixed vs FP64 at rel-L2<=1e-5; spread HBM GB/s",
@@ -163,9 +200,10 @@ def main():
         "stage_us": stage_us,
         "stage_GBps": stage_gbs,
         "spread_GBps": stage_gbs["spread"],
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": stage_gbs[dom],
+        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "achieved": stage_gbs[dom],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stage_gbs[dom] / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": sb[dom]},
+                     "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": sb[dom],
+                     "launch_us": stage_us[dom]},
     }
     del mixed
     if not args.no_fp64:
@@ -183,6 +221,8 @@ def main():
             w64 = W64.cpu().numpy()
             out["rel_l2_W_mixed_vs_fp64"] = float(np.linalg.norm(W32.double().cpu().numpy() - w64) / np.linalg.norm(w64))
         del f64
+    if args.copy_probe_mb > 0:
+        copy_probe(torch, args.copy_probe_mb)
     if args.cpu_budget > 0:
         rate, n, el, thr = cpu_baseline(wl, pos, F, T, args.cpu_budget)
         out["cpu_baseline"] = {"value": rate, "unit": "Mdot/s", "cores": thr, "kind": "port",
