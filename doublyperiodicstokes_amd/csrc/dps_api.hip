@@ -222,8 +222,8 @@ int build(dps_handle* h)
   }
   // tiles for spreading
   h->TX = std::min(16, h->Nx);
-  h->TY = std::min(4, h->Ny);
-  h->ZC = std::min(32, h->Nz);
+  h->TY = std::min(8, h->Ny);
+  h->ZC = std::min(h->prec == DPS_PREC_MIXED ? 32 : 16, h->Nz);
   h->ntx = (h->Nx + h->TX - 1) / h->TX;
   h->nty = (h->Ny + h->TY - 1) / h->TY;
   h->ntiles = h->ntx * h->nty;
@@ -253,11 +253,11 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc(&h->d_err, 1))) return rc;
   if (hipHostMalloc((void**)&h->h_err, sizeof(int)) != hipSuccess) return fail(DPS_ERR_ALLOC, "pinned alloc failed");
   if ((rc = h->alloc(&h->d_sup_m, N))) return rc;
-  h->rs_m = h->km.wx + h->km.wy + 2 * h->km.wzmax;
+  h->rs_m = h->km.wx + h->km.wy + 3 * h->km.wzmax;
   if ((rc = h->alloc((unsigned char**)&h->d_rec_m, (size_t)N * h->rs_m * ts))) return rc;
   if (h->has_dip) {
     if ((rc = h->alloc(&h->d_sup_d, N))) return rc;
-    h->rs_d = 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
+    h->rs_d = 2 * h->kd.wx + 2 * h->kd.wy + 5 * h->kd.wzmax;
     if ((rc = h->alloc((unsigned char**)&h->d_rec_d, (size_t)N * h->rs_d * ts))) return rc;
   }
   if ((rc = h->alloc(&h->d_zrange, N))) return rc;

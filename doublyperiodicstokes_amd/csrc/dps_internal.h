@@ -19,6 +19,14 @@ __host__ __device__ inline double kvec(int i, int n, double L)
   return 2 * M_PI * (double)nn / L;
 }
 
+// Block index -> work group index such that consecutive groups run on the
+// same XCD (the 8 XCDs receive blocks round robin and each has its own L2).
+__device__ __forceinline__ int xcd_group(int b, int nb)
+{
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 struct StencilArgs {
   int N;
   double xmin, ymin, zmin;

@@ -64,7 +64,6 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
     if (s.nz > K.wzmax) s.nz = K.wzmax;   // cannot happen (wzmax bounds the window)
     int4 sup;
     sup.x = s.bx; sup.y = s.by; sup.z = s.e0; sup.w = s.nz;
-    (kind ? A.sup_d : A.sup_m)[p] = sup;
     T* rec = (T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
     double wx[DPS_MAX_W], dx[DPS_MAX_W], wy[DPS_MAX_W], dy[DPS_MAX_W];
     stencil_weights_xy(x, A.hx, s.bx, K.wx, K, K.inv_norm3, wx, kind ? dx : nullptr);
@@ -80,34 +79,42 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
       for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
       o += K.wy;
     }
+    // z support, wall images folded in: one entry per support node that
+    // reaches the interior (open-end ghosts dropped), with its interior row
+    // and the fold factor already applied to the spread and interpolation
+    // weights (class A factor for the value kernel, class B for d/dz)
     const double a2 = K.alpha * K.alpha;
     const int W = K.wzmax;
+    const int* kt = kind ? A.kt_d : A.kt_m;
+    const double* fa = kind ? A.fA_d : A.fA_m;
+    const double* fb = kind ? A.fB_d : nullptr;
+    int nq = 0, klo = 1 << 30, khi = -1;
     for (int q = 0; q < s.nz; ++q) {
       const int e = s.e0 + q;
+      const double sa = fa[e], sb = kind ? fb[e] : 0.0;
+      if (sa == 0.0 && sb == 0.0) continue;
       const double v = clamp_off(zext[e] - z, K.thresh);
       const double ph = es_phi(v, K.betaw, a2);
       const double qw = hxhy * zwe[e];
-      rec[o + q] = (T)ph;                 // spread z weight
-      rec[o + W + q] = (T)(ph * qw);      // interpolation z weight
-      if (kind) {
+      if (kind == 0) {
+        rec[o + nq] = (T)(ph * sa);                 // spread
+        rec[o + W + nq] = (T)(ph * qw * sa);        // interpolation
+        rec[o + 2 * W + nq] = (T)kt[e];             // interior row
+      } else {
         const double d = ph * es_dfac(v, K.betaw, a2);
-        rec[o + 2 * W + q] = (T)d;
-        rec[o + 3 * W + q] = (T)(d * qw);
+        rec[o + nq] = (T)(ph * sa);
+        rec[o + W + nq] = (T)(d * sb);
+        rec[o + 2 * W + nq] = (T)(ph * qw * sa);
+        rec[o + 3 * W + nq] = (T)(d * qw * sb);
+        rec[o + 4 * W + nq] = (T)kt[e];
       }
+      klo = min(klo, kt[e]);
+      khi = max(khi, kt[e]);
+      ++nq;
     }
+    sup.w = nq;
+    (kind ? A.sup_d : A.sup_m)[p] = sup;
     {
-      // interior rows reached through this support (after the wall fold)
-      int klo = 1 << 30, khi = -1;
-      const int* kt = kind ? A.kt_d : A.kt_m;
-      const double* fa = kind ? A.fA_d : A.fA_m;
-      const double* fb = kind ? A.fB_d : nullptr;
-      for (int q = 0; q < s.nz; ++q) {
-        const int e = s.e0 + q;
-        if (fa[e] != 0.0 || (kind && fb[e] != 0.0)) {
-          klo = min(klo, kt[e]);
-          khi = max(khi, kt[e]);
-        }
-      }
       int4 zr = A.zrange[p];
       if (kind) { zr.z = klo; zr.w = khi; }
       else { zr.x = klo; zr.y = khi; zr.z = 1 << 30; zr.w = -1; }
@@ -181,108 +188,133 @@ __device__ __forceinline__ int clip_piece(int b, int w, int t0, int n, int perio
   return hi - lo;
 }
 
+// t = col * n + q for t < 2^24: quotient and remainder by a float reciprocal
+// with one correction step (no integer division in the point loops)
+__device__ __forceinline__ int fdiv(int t, int n, float inv, int* rem)
+{
+  int c = (int)((float)t * inv);
+  int r = t - c * n;
+  if (r < 0) { c -= 1; r += n; }
+  else if (r >= n) { c += 1; r -= n; }
+  *rem = r;
+  return c;
+}
+
+// Spreading.  One workgroup owns a tile of TX x TY columns and ZC interior z
+// rows.  The particles of the tile's neighbourhood are culled in parallel (one
+// thread per candidate) into an LDS queue of (particle, kernel) entries whose
+// folded support reaches the tile; each wavefront then takes queue entries
+// and spreads every (column, z) point of the entry's clipped support with one
+// lane per point, accumulating with LDS atomics.  Every grid element of the
+// tile is written to HBM once, coalesced.
 template <class T, class Tin>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restrict__ F,
                                                 const Tin* __restrict__ Tq)
 {
-  extern __shared__ unsigned char smem_raw[];
+  // one dynamic LDS region, no static LDS: the accumulator must start at
+  // offset 0 so that FP64 accumulators are 8-byte aligned for ds_add_f64
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* acc = reinterpret_cast<T*>(smem_raw);
+  const int plane = A.TY * A.TX, vol = A.ZC * plane;
+  // entries p * 2 + kind; at most 2 per thread per culling round
+  int* queue = reinterpret_cast<int*>(acc + 3 * vol);
+  int& qn = queue[2 * 256];
   T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
   const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
   const int X0 = tx * A.TX, Y0 = ty * A.TY, Z0 = zc * A.ZC;
   const int nX = min(A.TX, A.Nx - X0), nY = min(A.TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
-  const int plane = A.TY * A.TX, vol = A.ZC * plane;
   for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (T)0;
+  if (threadIdx.x == 0) qn = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  // neighbour tiles (unique after periodic wrap)
+  const int nkind = Tq ? 2 : 1;
   const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
   const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
-  int job = 0;   // running particle counter used to deal particles to waves
-  for (int dyt = 0; dyt < nby; ++dyt) {
-    const int ty2 = wrapi(by0 + dyt, A.nty);
-    for (int dxt = 0; dxt < nbx; ++dxt) {
-      const int t2 = ty2 * A.ntx + wrapi(bx0 + dxt, A.ntx);
-      const int pb = A.start[t2], pe = A.start[t2 + 1];
-      for (int pi = pb; pi < pe; ++pi, ++job) {
-        if ((job % nwaves) != wave) continue;
+  for (int nt = 0; nt < nbx * nby; ++nt) {
+    const int t2 = wrapi(by0 + nt / nbx, A.nty) * A.ntx + wrapi(bx0 + nt % nbx, A.ntx);
+    const int pb = A.start[t2], pe = A.start[t2 + 1];
+    for (int c0 = pb; c0 < pe; c0 += blockDim.x) {
+      // ---- cull: one candidate particle per thread ----
+      const int pi = c0 + threadIdx.x;
+      if (pi < pe) {
         const int p = A.order[pi];
         const int4 zr = A.zrange[p];
-        for (int kind = 0; kind < (Tq ? 2 : 1); ++kind) {
-          // interior z rows this particle touches
+        for (int kind = 0; kind < nkind; ++kind) {
           const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
           if (khi < Z0 || klo >= Z0 + nZ) continue;
           const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
-          const KernelDesc& K = kind ? A.kd : A.km;
-          const int* kt = kind ? A.kt_d : A.kt_m;
-          const double* fA = kind ? A.fA_d : A.fA_m;
-          const double* fB = kind ? A.fB_d : A.fB_m;
-          const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
-          const T* wx = rec;
-          const T* wy = rec + K.wx;
-          const T* dxw = rec + K.wx + K.wy;
-          const T* dyw = dxw + K.wx;
-          const T* wz = kind ? (dyw + K.wy) : (rec + K.wx + K.wy);
-          const T* dzw = wz + 2 * K.wzmax;
-          T f0, f1, f2;
-          if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
-          else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
-          const int nzq = sup.w;
-          for (int sxp = -1; sxp <= 1; ++sxp) {
-            int ix0, lx0;
-            const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
-            if (!cx) continue;
-            for (int syp = -1; syp <= 1; ++syp) {
-              int iy0, ly0;
-              const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
-              if (!cy) continue;
-              // one lane per support column, walking the z support
-              for (int cb = 0; cb < cx * cy; cb += 64) {
-                const int cidx = cb + lane;
-                if (cidx >= cx * cy) continue;
-                const int a = cidx % cx, b = cidx / cx;
-                const int i = ix0 + a, j = iy0 + b;
-                T* col = acc + (ly0 + b) * A.TX + (lx0 + a);
-                if (kind == 0) {
-                  const T wxy = wx[i] * wy[j];
-                  for (int q = 0; q < nzq; ++q) {
-                    const int e = sup.z + q;
-                    const int k = kt[e] - Z0;
-                    if (k < 0 || k >= nZ) continue;
-                    const T fa = (T)fA[e];
-                    if (fa == (T)0) continue;
-                    const T w = wxy * wz[q] * fa;
-                    T* dst = col + k * plane;
-                    atomicAdd(dst, w * f0);
-                    atomicAdd(dst + vol, w * f1);
-                    atomicAdd(dst + 2 * vol, w * f2);
-                  }
-                } else {
-                  const T gxy = dxw[i] * wy[j], gyx = wx[i] * dyw[j], wxy = wx[i] * wy[j];
-                  const T h = (T)0.5;
-                  for (int q = 0; q < nzq; ++q) {
-                    const int e = sup.z + q;
-                    const int k = kt[e] - Z0;
-                    if (k < 0 || k >= nZ) continue;
-                    const T fa = (T)fA[e], fb = (T)fB[e];
-                    if (fa == (T)0 && fb == (T)0) continue;
-                    const T gx = gxy * wz[q] * fa;
-                    const T gy = gyx * wz[q] * fa;
-                    const T gz = wxy * dzw[q] * fb;
-                    T* dst = col + k * plane;
-                    atomicAdd(dst, h * (gy * f2 - gz * f1));
-                    atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
-                    atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
-                  }
-                }
+          const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
+          bool hx = false, hy = false;
+          int i0, l0;
+          for (int sh = -1; sh <= 1; ++sh) {
+            hx = hx || clip_piece(sup.x, wx, X0, nX, A.Nx, sh * A.Nx, &i0, &l0) > 0;
+            hy = hy || clip_piece(sup.y, wy, Y0, nY, A.Ny, sh * A.Ny, &i0, &l0) > 0;
+          }
+          if (hx && hy) queue[atomicAdd(&qn, 1)] = 2 * p + kind;
+        }
+      }
+      __syncthreads();
+      // ---- spread the queued entries ----
+      const int nq_entries = qn;
+      for (int qi = wave; qi < nq_entries; qi += nwaves) {
+        const int ent = queue[qi];
+        const int p = ent >> 1, kind = ent & 1;
+        const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
+        const KernelDesc& K = kind ? A.kd : A.km;
+        const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
+        const int W = K.wzmax, nz = sup.w;
+        const T* wxr = rec;
+        const T* wyr = rec + K.wx;
+        const T* dxr = wyr + K.wy;
+        const T* dyr = dxr + K.wx;
+        const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);
+        const T* rowr = zr + (kind ? 4 : 2) * W;
+        T f0, f1, f2;
+        if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
+        else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
+        const float inz = 1.0f / (float)nz;
+        for (int sxp = -1; sxp <= 1; ++sxp) {
+          int ix0, lx0;
+          const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
+          if (!cx) continue;
+          const float icx = 1.0f / (float)cx;
+          for (int syp = -1; syp <= 1; ++syp) {
+            int iy0, ly0;
+            const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
+            if (!cy) continue;
+            const int npts = cx * cy * nz;
+            for (int t = lane; t < npts; t += 64) {
+              int q, a;
+              const int col = fdiv(t, nz, inz, &q);
+              const int b = fdiv(col, cx, icx, &a);
+              const int k = (int)rowr[q] - Z0;
+              if (k < 0 || k >= nZ) continue;
+              T* dst = acc + k * plane + (ly0 + b) * A.TX + (lx0 + a);
+              const int i = ix0 + a, j = iy0 + b;
+              if (kind == 0) {
+                const T w = wxr[i] * wyr[j] * zr[q];
+                atomicAdd(dst, w * f0);
+                atomicAdd(dst + vol, w * f1);
+                atomicAdd(dst + 2 * vol, w * f2);
+              } else {
+                const T wzA = zr[q], dzB = zr[W + q];
+                const T gx = dxr[i] * wyr[j] * wzA;
+                const T gy = wxr[i] * dyr[j] * wzA;
+                const T gz = wxr[i] * wyr[j] * dzB;
+                const T h = (T)0.5;
+                atomicAdd(dst, h * (gy * f2 - gz * f1));
+                atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
+                atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
               }
             }
           }
         }
       }
+      __syncthreads();
+      if (threadIdx.x == 0) qn = 0;
+      __syncthreads();
     }
   }
-  __syncthreads();
   // one write per grid element of the tile
   const int nrows = 3 * nZ * nY;
   for (int t = threadIdx.x; t < nrows * A.TX; t += blockDim.x) {
@@ -309,93 +341,76 @@ __device__ __forceinline__ T wave_sum(T v)
   return v;
 }
 
+// Interpolation: one wavefront per particle (tile order, mapped so that each
+// XCD handles a contiguous run of tiles), one lane per (column, z) point of
+// the folded support, wavefront sums.  Ω from the derivative kernels:
+// omega = -1/2 (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)  (FCM.py:424-426)
 template <class T, class Tout>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__ vel,
                                                 Tout* __restrict__ ang)
 {
-  const int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (g >= A.N) return;
   const int p = A.order[g];
   const T* __restrict__ U = reinterpret_cast<const T*>(A.grid);
   const size_t cs = (size_t)A.Nz * A.Ny * A.Nx;
-  {
-    const int4 sup = A.sup_m[p];
-    const KernelDesc& K = A.km;
-    const T* rec = (const T*)A.rec_m + (size_t)p * A.rs_m;
-    const T* wx = rec;
-    const T* wy = rec + K.wx;
-    const T* wzi = rec + K.wx + K.wy + K.wzmax;
-    T u0 = 0, u1 = 0, u2 = 0;
-    for (int cb = 0; cb < K.wx * K.wy; cb += 64) {
-      const int cidx = cb + lane;
-      if (cidx < K.wx * K.wy) {
-        const int a = cidx % K.wx, b = cidx / K.wx;
-        const size_t col = (size_t)wrapi(sup.y + b, A.Ny) * A.Nx + wrapi(sup.x + a, A.Nx);
-        const T wxy = wx[a] * wy[b];
-        for (int q = 0; q < sup.w; ++q) {
-          const int e = sup.z + q;
-          const T fa = (T)A.fA_m[e];
-          if (fa == (T)0) continue;
-          const size_t idx = (size_t)A.kt_m[e] * A.Ny * A.Nx + col;
-          const T w = wxy * wzi[q] * fa;
-          u0 += w * U[idx];
-          u1 += w * U[idx + cs];
-          u2 += w * U[idx + 2 * cs];
-        }
-      }
-    }
-    u0 = wave_sum(u0);
-    u1 = wave_sum(u1);
-    u2 = wave_sum(u2);
-    if (lane == 0) {
-      vel[3 * p] = (Tout)u0;
-      vel[3 * p + 1] = (Tout)u1;
-      vel[3 * p + 2] = (Tout)u2;
-    }
-  }
-  if (!ang) return;
-  {
-    const int4 sup = A.sup_d[p];
-    const KernelDesc& K = A.kd;
-    const T* rec = (const T*)A.rec_d + (size_t)p * A.rs_d;
-    const T* wx = rec;
-    const T* wy = rec + K.wx;
-    const T* dxw = wy + K.wy;
-    const T* dyw = dxw + K.wx;
-    const T* wzi = dyw + K.wy + K.wzmax;
-    const T* dzi = dyw + K.wy + 3 * K.wzmax;
+  const size_t zs = (size_t)A.Ny * A.Nx;
+  for (int kind = 0; kind < (ang ? 2 : 1); ++kind) {
+    const int4 sup = kind ? A.sup_d[p] : A.sup_m[p];
+    const KernelDesc& K = kind ? A.kd : A.km;
+    const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
+    const int W = K.wzmax, nz = sup.w, wx = K.wx, wy = K.wy;
+    const T* wxr = rec;
+    const T* wyr = rec + wx;
+    const T* dxr = wyr + wy;
+    const T* dyr = dxr + wx;
+    const T* zr = kind ? (dyr + wy) : (wyr + wy);
+    const T* wzi = zr + (kind ? 2 : 1) * W;   // interpolation weights (value kernel)
+    const T* dzi = zr + 3 * W;                // d/dz kernel (dipole only)
+    const T* rowr = zr + (kind ? 4 : 2) * W;
+    const int npts = wx * wy * nz;
+    const float inz = 1.0f / (float)nz, iwx = 1.0f / (float)wx;
     T d[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = 0;
-    for (int cb = 0; cb < K.wx * K.wy; cb += 64) {
-      const int cidx = cb + lane;
-      if (cidx < K.wx * K.wy) {
-        const int a = cidx % K.wx, b = cidx / K.wx;
-        const size_t col = (size_t)wrapi(sup.y + b, A.Ny) * A.Nx + wrapi(sup.x + a, A.Nx);
-        const T gxy = dxw[a] * wy[b], gyx = wx[a] * dyw[b], wxy = wx[a] * wy[b];
-        for (int q = 0; q < sup.w; ++q) {
-          const int e = sup.z + q;
-          const T fa = (T)A.fA_d[e], fb = (T)A.fB_d[e];
-          if (fa == (T)0 && fb == (T)0) continue;
-          const size_t idx = (size_t)A.kt_d[e] * A.Ny * A.Nx + col;
-          const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
-          const T gx = gxy * wzi[q] * fa;
-          const T gy = gyx * wzi[q] * fa;
-          const T gz = wxy * dzi[q] * fb;
-          d[0] += gx * v0; d[1] += gx * v1; d[2] += gx * v2;
-          d[3] += gy * v0; d[4] += gy * v1; d[5] += gy * v2;
-          d[6] += gz * v0; d[7] += gz * v1; d[8] += gz * v2;
-        }
+    for (int t = lane; t < npts; t += 64) {
+      int q, a;
+      const int col = fdiv(t, nz, inz, &q);
+      const int b = fdiv(col, wx, iwx, &a);
+      int x = sup.x + a, y = sup.y + b;
+      x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+      y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+      const size_t idx = (size_t)(int)rowr[q] * zs + (size_t)y * A.Nx + x;
+      const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
+      if (kind == 0) {
+        const T w = wxr[a] * wyr[b] * wzi[q];
+        d[0] += w * v0; d[1] += w * v1; d[2] += w * v2;
+      } else {
+        const T gx = dxr[a] * wyr[b] * wzi[q];
+        const T gy = wxr[a] * dyr[b] * wzi[q];
+        const T gz = wxr[a] * wyr[b] * dzi[q];
+        d[0] += gx * v0; d[1] += gx * v1; d[2] += gx * v2;
+        d[3] += gy * v0; d[4] += gy * v1; d[5] += gy * v2;
+        d[6] += gz * v0; d[7] += gz * v1; d[8] += gz * v2;
       }
     }
+    if (kind == 0) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) d[i] = wave_sum(d[i]);
-    if (lane == 0) {
-      // omega = -1/2 (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)  (FCM.py:424-426)
-      ang[3 * p] = (Tout)((T)(-0.5) * (d[5] - d[7]));
-      ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[6] - d[2]));
-      ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[1] - d[3]));
+      for (int i = 0; i < 3; ++i) d[i] = wave_sum(d[i]);
+      if (lane == 0) {
+        vel[3 * p] = (Tout)d[0];
+        vel[3 * p + 1] = (Tout)d[1];
+        vel[3 * p + 2] = (Tout)d[2];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) d[i] = wave_sum(d[i]);
+      if (lane == 0) {
+        ang[3 * p] = (Tout)((T)(-0.5) * (d[5] - d[7]));
+        ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[6] - d[2]));
+        ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[1] - d[3]));
+      }
     }
   }
 }
@@ -413,7 +428,7 @@ __global__ void __launch_bounds__(256) k_zfwd(ZArgs A, const typename C2<T>::t* 
 {
   typedef typename C2<T>::t V;
   constexpr int CW = ZCols<T>::value, RS = 256 / CW;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   V* E = reinterpret_cast<V*>(smem_raw);            // [NH][CW]
   V* O = E + (size_t)A.NH * CW;                      // [NH][CW]
   const int c = blockIdx.y;
@@ -483,7 +498,7 @@ __global__ void __launch_bounds__(256) k_zinv(ZArgs A, const typename C2<T>::t* 
 {
   typedef typename C2<T>::t V;
   constexpr int CW = ZCols<T>::value, RS = 256 / CW;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   V* Ck = reinterpret_cast<V*>(smem_raw);   // [N][CW]
   const int c = blockIdx.y;
   const int m0 = blockIdx.x * CW;
@@ -777,7 +792,7 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
 template <class T, class Tin>
 void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s)
 {
-  const size_t lds = (size_t)3 * A.ZC * A.TY * A.TX * sizeof(T);
+  const size_t lds = (size_t)3 * A.ZC * A.TY * A.TX * sizeof(T) + (2 * 256 + 4) * sizeof(int);
   dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
   hipLaunchKernelGGL((k_spread<T, Tin>), grid, dim3(256), lds, s, A, (const Tin*)F,
                      (const Tin*)Tq);

@@ -126,22 +126,13 @@ struct GlobalCoef {
   }
 };
 
-// Block index -> mode group: consecutive groups run on the same XCD (blocks
-// are dispatched to the 8 XCDs round robin), so the two halves of a 128-byte
-// line shared by neighbouring groups meet in one L2.
-__device__ __forceinline__ int xcd_group(int b, int nb)
-{
-  const int x = b & 7, q = nb >> 3, r = nb & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 template <class T, int LPM, int E, int NRV>
 __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<T>::t* __restrict__ in,
                                                   typename C2<T>::t* __restrict__ out)
 {
   typedef typename C2<T>::t V;
   constexpr int R = E / 2;   // node pairs per lane in the inverse transform
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int Nz = A.B.Nz, G = A.G, Nm = A.Nm, NH = A.NH;
   const int ncol = 3 * Nz * G;
   V* S = reinterpret_cast<V*>(smem_raw);   // [c][row][g]: values, coefficients, values
