@@ -229,6 +229,20 @@ int build(dps_handle* h)
   h->ntiles = h->ntx * h->nty;
   h->RX = (wmax + 1 + h->TX - 1) / h->TX + ((h->Nx % h->TX) ? 1 : 0);
   h->RY = (wmax + 1 + h->TY - 1) / h->TY + ((h->Ny % h->TY) ? 1 : 0);
+  if (std::min(2 * h->RX + 1, h->ntx) * std::min(2 * h->RY + 1, h->nty) > DPS_SPREAD_MAXT)
+    return fail(DPS_ERR_ARG, "kernel support too wide for the spreading tiles");
+  // particle weight records: mono [wx][wy][z spread][z interp][z row];
+  // dipole [wx][wy][dx][dy][z spread][dz spread][z interp][dz interp][z row]
+  h->rs_m = h->km.wx + h->km.wy + 3 * h->km.wzmax;
+  h->rs_d = h->has_dip ? 2 * h->kd.wx + 2 * h->kd.wy + 5 * h->kd.wzmax : 0;
+  {
+    // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
+    SpreadArgs S{};
+    S.TX = h->TX; S.TY = h->TY; S.ZC = h->ZC; S.rs_max = std::max(h->rs_m, h->rs_d);
+    while (spread_lds_bytes(S, h->tsz) > 65536 && S.ZC > 1) S.ZC /= 2;
+    if (spread_lds_bytes(S, h->tsz) > 65536) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
+    h->ZC = S.ZC;
+  }
   return DPS_OK;
 }
 
@@ -253,11 +267,9 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc(&h->d_err, 1))) return rc;
   if (hipHostMalloc((void**)&h->h_err, sizeof(int)) != hipSuccess) return fail(DPS_ERR_ALLOC, "pinned alloc failed");
   if ((rc = h->alloc(&h->d_sup_m, N))) return rc;
-  h->rs_m = h->km.wx + h->km.wy + 3 * h->km.wzmax;
   if ((rc = h->alloc((unsigned char**)&h->d_rec_m, (size_t)N * h->rs_m * ts))) return rc;
   if (h->has_dip) {
     if ((rc = h->alloc(&h->d_sup_d, N))) return rc;
-    h->rs_d = 2 * h->kd.wx + 2 * h->kd.wy + 5 * h->kd.wzmax;
     if ((rc = h->alloc((unsigned char**)&h->d_rec_d, (size_t)N * h->rs_d * ts))) return rc;
   }
   if ((rc = h->alloc(&h->d_zrange, N))) return rc;
@@ -486,8 +498,7 @@ SpreadArgs spread_args(dps_handle* h)
   A.km = h->km; A.kd = h->kd;
   A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
   A.rec_m = h->d_rec_m; A.rec_d = h->d_rec_d; A.rs_m = h->rs_m; A.rs_d = h->rs_d;
-  A.kt_m = h->d_kt_m; A.kt_d = h->d_kt_d;
-  A.fA_m = h->d_fA_m; A.fB_m = h->d_fB_m; A.fA_d = h->d_fA_d; A.fB_d = h->d_fB_d;
+  A.rs_max = std::max(h->rs_m, h->rs_d);
   A.grid = h->d_grid;
   return A;
 }

@@ -8,10 +8,12 @@
 #include <math.h>
 #include <stdint.h>
 
+// Everything here is small and called from hot loops: always inlined, so that
+// arrays passed by reference stay in registers.
 #if defined(__HIPCC__)
-#define DPS_HD __host__ __device__ inline
+#define DPS_HD __host__ __device__ inline __attribute__((always_inline))
 #else
-#define DPS_HD inline
+#define DPS_HD inline __attribute__((always_inline))
 #endif
 
 namespace dps {
@@ -32,6 +34,11 @@ DPS_HD cplx operator*(cplx a, double s) { return mk(a.re * s, a.im * s); }
 DPS_HD cplx operator/(cplx a, double s) { return mk(a.re / s, a.im / s); }
 DPS_HD cplx conjg(cplx a) { return mk(a.re, -a.im); }
 DPS_HD cplx times_i(cplx a) { return mk(-a.im, a.re); }   // i * a
+// Selects on values.  (A conditional expression on two lvalues is an lvalue:
+// the compiler then selects between their addresses, which keeps the
+// operands in memory instead of registers.)
+DPS_HD double dsel(bool s, double a, double b) { return s ? a : b; }
+DPS_HD cplx csel(bool s, cplx a, cplx b) { return mk(s ? a.re : b.re, s ? a.im : b.im); }
 
 // geometry codes (same values as the C ABI)
 enum { MODE_NOWALL = 1, MODE_BOTTOM = 2, MODE_SLIT = 3 };
@@ -627,6 +634,83 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
 #undef AB
 #undef PP
 
+// One row of a real 4x4 system with a complex right-hand side.  The solve
+// below keeps every row in named scalars (no arrays, no runtime indices), so
+// on the GPU the system lives in registers.
+struct Row4 {
+  double a0, a1, a2, a3;
+  cplx b;
+};
+
+DPS_HD void row_swap_if(bool s, Row4& x, Row4& y)
+{
+  const Row4 t = x, u = y;
+  x.a0 = dsel(s, u.a0, t.a0); y.a0 = dsel(s, t.a0, u.a0);
+  x.a1 = dsel(s, u.a1, t.a1); y.a1 = dsel(s, t.a1, u.a1);
+  x.a2 = dsel(s, u.a2, t.a2); y.a2 = dsel(s, t.a2, u.a2);
+  x.a3 = dsel(s, u.a3, t.a3); y.a3 = dsel(s, t.a3, u.a3);
+  x.b = csel(s, u.b, t.b);    y.b = csel(s, t.b, u.b);
+}
+
+// y -= (y[col] / x[col]) x for the columns after `col` (Gaussian elimination step)
+DPS_HD void row_eliminate(const Row4& x, Row4& y, int col, double inv)
+{
+  if (col == 0) {
+    const double l = y.a0 * inv;
+    y.a1 -= l * x.a1; y.a2 -= l * x.a2; y.a3 -= l * x.a3; y.b = y.b - l * x.b;
+  } else if (col == 1) {
+    const double l = y.a1 * inv;
+    y.a2 -= l * x.a2; y.a3 -= l * x.a3; y.b = y.b - l * x.b;
+  } else {
+    const double l = y.a2 * inv;
+    y.a3 -= l * x.a3; y.b = y.b - l * x.b;
+  }
+}
+
+// Gaussian elimination with partial pivoting (largest |entry| of the column,
+// first one on ties) and back substitution; x receives the solution.
+DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx* x)
+{
+  // column 0
+  {
+    int piv = 0;
+    double best = fabs(R0.a0);
+    if (fabs(R1.a0) > best) { best = fabs(R1.a0); piv = 1; }
+    if (fabs(R2.a0) > best) { best = fabs(R2.a0); piv = 2; }
+    if (fabs(R3.a0) > best) { best = fabs(R3.a0); piv = 3; }
+    row_swap_if(piv == 1, R0, R1);
+    row_swap_if(piv == 2, R0, R2);
+    row_swap_if(piv == 3, R0, R3);
+    const double inv = 1.0 / R0.a0;
+    row_eliminate(R0, R1, 0, inv);
+    row_eliminate(R0, R2, 0, inv);
+    row_eliminate(R0, R3, 0, inv);
+  }
+  // column 1
+  {
+    int piv = 1;
+    double best = fabs(R1.a1);
+    if (fabs(R2.a1) > best) { best = fabs(R2.a1); piv = 2; }
+    if (fabs(R3.a1) > best) { best = fabs(R3.a1); piv = 3; }
+    row_swap_if(piv == 2, R1, R2);
+    row_swap_if(piv == 3, R1, R3);
+    const double inv = 1.0 / R1.a1;
+    row_eliminate(R1, R2, 1, inv);
+    row_eliminate(R1, R3, 1, inv);
+  }
+  // column 2
+  {
+    const bool sw = fabs(R3.a2) > fabs(R2.a2);
+    row_swap_if(sw, R2, R3);
+    const double inv = 1.0 / R2.a2;
+    row_eliminate(R2, R3, 2, inv);
+  }
+  x[3] = R3.b / R3.a3;
+  x[2] = (R2.b - R2.a3 * x[3]) / R2.a2;
+  x[1] = ((R1.b - R1.a2 * x[2]) - R1.a3 * x[3]) / R1.a1;
+  x[0] = (((R0.b - R0.a1 * x[1]) - R0.a2 * x[2]) - R0.a3 * x[3]) / R0.a0;
+}
+
 DPS_HD void wall_correction(const BvpConsts& B, double kx, double ky, const cplx* wall_bot,
                             const cplx* wall_top, ModeResult& res)
 {
@@ -658,33 +742,12 @@ DPS_HD void wall_correction(const BvpConsts& B, double kx, double ky, const cplx
     // rows 0,1,4,7 of the 8x8 system couple only (x0, x1, x6, x7) and are
     // real: solve that 4x4 block with partial pivoting, then the two 2x2
     // blocks for (x2, x3) and (x4, x5).
-    double A[4][4] = {{fac, enkh * fac, -k, k * enkh},
-                      {(1 - k * Hs) * enkh * fac, (1 + k * Hs) * fac, -k * enkh, k},
-                      {0, 0, 1, enkh},
-                      {Hs * enkh * fac, Hs * fac, enkh, 1}};
-    cplx b4[4] = {rhs[0], rhs[1], rhs[4], rhs[7]};
-    for (int col = 0; col < 4; ++col) {
-      int piv = col;
-      double best = fabs(A[col][col]);
-      for (int r = col + 1; r < 4; ++r)
-        if (fabs(A[r][col]) > best) { best = fabs(A[r][col]); piv = r; }
-      if (piv != col) {
-        for (int cc = 0; cc < 4; ++cc) { double t = A[col][cc]; A[col][cc] = A[piv][cc]; A[piv][cc] = t; }
-        cplx t = b4[col]; b4[col] = b4[piv]; b4[piv] = t;
-      }
-      const double inv = 1.0 / A[col][col];
-      for (int r = col + 1; r < 4; ++r) {
-        const double l = A[r][col] * inv;
-        for (int cc = col + 1; cc < 4; ++cc) A[r][cc] -= l * A[col][cc];
-        b4[r] = b4[r] - l * b4[col];
-      }
-    }
+    Row4 R0 = {fac, enkh * fac, -k, k * enkh, rhs[0]};
+    Row4 R1 = {(1 - k * Hs) * enkh * fac, (1 + k * Hs) * fac, -k * enkh, k, rhs[1]};
+    Row4 R2 = {0, 0, 1, enkh, rhs[4]};
+    Row4 R3 = {Hs * enkh * fac, Hs * fac, enkh, 1, rhs[7]};
     cplx xs4[4];
-    for (int r = 3; r >= 0; --r) {
-      cplx s = b4[r];
-      for (int cc = r + 1; cc < 4; ++cc) s = s - A[r][cc] * xs4[cc];
-      xs4[r] = s / A[r][r];
-    }
+    solve4_pivot(R0, R1, R2, R3, xs4);
     const cplx X0 = xs4[0], X1 = xs4[1], X6 = xs4[2], X7 = xs4[3];
     // rows 2,5: x2 + enkh x3 = b2 ; enkh x2 + x3 = b5 - (A50 X0 + A51 X1)
     const double d2 = 1 - enkh * enkh;

@@ -4,7 +4,8 @@
 #include <hip/hip_runtime.h>
 #include "dps_core.h"
 
-#define DPS_MAX_W 32   // largest x/y support (points) a stencil may have
+#define DPS_MAX_W 32          // largest x/y support (points) a stencil may have
+#define DPS_SPREAD_MAXT 128   // largest number of neighbour tiles a spreading workgroup scans
 
 namespace dps {
 
@@ -56,11 +57,10 @@ struct SpreadArgs {
   KernelDesc km, kd;
   const int4 *sup_m, *sup_d;
   const void *rec_m, *rec_d;
-  int rs_m, rs_d;
-  const int *kt_m, *kt_d;
-  const double *fA_m, *fB_m, *fA_d, *fB_d;
+  int rs_m, rs_d, rs_max;
   void* grid;   // [3][Nz][Ny][Nx] grid precision
 };
+size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
 
 struct InterpArgs {
   int N, Nx, Ny, Nz;

@@ -211,122 +211,151 @@ template <class T, class Tin>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restrict__ F,
                                                 const Tin* __restrict__ Tq)
 {
+  constexpr int MAXT = DPS_SPREAD_MAXT;   // neighbour tiles a workgroup scans (checked on the host)
   // one dynamic LDS region, no static LDS: the accumulator must start at
   // offset 0 so that FP64 accumulators are 8-byte aligned for ds_add_f64
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   T* acc = reinterpret_cast<T*>(smem_raw);
-  const int plane = A.TY * A.TX, vol = A.ZC * plane;
-  // entries p * 2 + kind; at most 2 per thread per culling round
-  int* queue = reinterpret_cast<int*>(acc + 3 * vol);
-  int& qn = queue[2 * 256];
+  // padded tile [c][z][y][x]: row stride TX+1, odd plane stride, so that the
+  // lanes of one atomic instruction spread over the LDS banks
+  const int RS = A.TX + 1, PL = A.TY * RS + 1, vol = A.ZC * PL;
+  T* stage = acc + 3 * vol;                                       // [wave][rs] weight records
+  int* queue = reinterpret_cast<int*>(stage + (blockDim.x >> 6) * A.rs_max);   // p * 2 + kind
+  int* tstart = queue + 2 * 256;   // first candidate of each neighbour tile
+  int* tpre = tstart + MAXT;       // prefix sums of the tiles' counts
+  int& qn = tpre[MAXT + 1];
   T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
   const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
   const int X0 = tx * A.TX, Y0 = ty * A.TY, Z0 = zc * A.ZC;
   const int nX = min(A.TX, A.Nx - X0), nY = min(A.TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
   for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (T)0;
-  if (threadIdx.x == 0) qn = 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const int nkind = Tq ? 2 : 1;
   const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
   const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
-  for (int nt = 0; nt < nbx * nby; ++nt) {
-    const int t2 = wrapi(by0 + nt / nbx, A.nty) * A.ntx + wrapi(bx0 + nt % nbx, A.ntx);
-    const int pb = A.start[t2], pe = A.start[t2 + 1];
-    for (int c0 = pb; c0 < pe; c0 += blockDim.x) {
-      // ---- cull: one candidate particle per thread ----
-      const int pi = c0 + threadIdx.x;
-      if (pi < pe) {
-        const int p = A.order[pi];
-        const int4 zr = A.zrange[p];
-        for (int kind = 0; kind < nkind; ++kind) {
-          const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
-          if (khi < Z0 || klo >= Z0 + nZ) continue;
-          const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
-          const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
-          bool hx = false, hy = false;
-          int i0, l0;
-          for (int sh = -1; sh <= 1; ++sh) {
-            hx = hx || clip_piece(sup.x, wx, X0, nX, A.Nx, sh * A.Nx, &i0, &l0) > 0;
-            hy = hy || clip_piece(sup.y, wy, Y0, nY, A.Ny, sh * A.Ny, &i0, &l0) > 0;
-          }
-          if (hx && hy) queue[atomicAdd(&qn, 1)] = 2 * p + kind;
-        }
-      }
-      __syncthreads();
-      // ---- spread the queued entries ----
-      const int nq_entries = qn;
-      for (int qi = wave; qi < nq_entries; qi += nwaves) {
-        const int ent = queue[qi];
-        const int p = ent >> 1, kind = ent & 1;
+  const int ntl = nbx * nby;
+  if (threadIdx.x == 0) {
+    // the neighbourhood's candidates as one list: tile j holds candidates
+    // [tpre[j], tpre[j+1]) starting at order position tstart[j]
+    int run = 0;
+    for (int j = 0; j < ntl; ++j) {
+      const int t2 = wrapi(by0 + j / nbx, A.nty) * A.ntx + wrapi(bx0 + j % nbx, A.ntx);
+      tstart[j] = A.start[t2];
+      tpre[j] = run;
+      run += A.start[t2 + 1] - A.start[t2];
+    }
+    tpre[ntl] = run;
+    qn = 0;
+  }
+  __syncthreads();
+  const int ncand = tpre[ntl];
+  T* st = stage + wave * A.rs_max;
+  for (int c0 = 0; c0 < ncand; c0 += blockDim.x) {
+    // ---- cull: one candidate particle per thread ----
+    const int ci = c0 + threadIdx.x;
+    if (ci < ncand) {
+      int j = 0;
+      while (j + 1 < ntl && tpre[j + 1] <= ci) ++j;
+      const int p = A.order[tstart[j] + (ci - tpre[j])];
+      const int4 zr = A.zrange[p];
+      for (int kind = 0; kind < nkind; ++kind) {
+        const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
+        if (khi < Z0 || klo >= Z0 + nZ) continue;
         const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
-        const KernelDesc& K = kind ? A.kd : A.km;
-        const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
-        const int W = K.wzmax, nz = sup.w;
-        const T* wxr = rec;
-        const T* wyr = rec + K.wx;
-        const T* dxr = wyr + K.wy;
-        const T* dyr = dxr + K.wx;
-        const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);
-        const T* rowr = zr + (kind ? 4 : 2) * W;
-        T f0, f1, f2;
-        if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
-        else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
-        const float inz = 1.0f / (float)nz;
-        for (int sxp = -1; sxp <= 1; ++sxp) {
-          int ix0, lx0;
-          const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
-          if (!cx) continue;
-          const float icx = 1.0f / (float)cx;
-          for (int syp = -1; syp <= 1; ++syp) {
-            int iy0, ly0;
-            const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
-            if (!cy) continue;
-            const int npts = cx * cy * nz;
-            for (int t = lane; t < npts; t += 64) {
-              int q, a;
-              const int col = fdiv(t, nz, inz, &q);
-              const int b = fdiv(col, cx, icx, &a);
-              const int k = (int)rowr[q] - Z0;
-              if (k < 0 || k >= nZ) continue;
-              T* dst = acc + k * plane + (ly0 + b) * A.TX + (lx0 + a);
-              const int i = ix0 + a, j = iy0 + b;
-              if (kind == 0) {
-                const T w = wxr[i] * wyr[j] * zr[q];
-                atomicAdd(dst, w * f0);
-                atomicAdd(dst + vol, w * f1);
-                atomicAdd(dst + 2 * vol, w * f2);
-              } else {
-                const T wzA = zr[q], dzB = zr[W + q];
-                const T gx = dxr[i] * wyr[j] * wzA;
-                const T gy = wxr[i] * dyr[j] * wzA;
-                const T gz = wxr[i] * wyr[j] * dzB;
-                const T h = (T)0.5;
-                atomicAdd(dst, h * (gy * f2 - gz * f1));
-                atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
-                atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
-              }
+        const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
+        bool hx = false, hy = false;
+        int i0, l0;
+        for (int sh = -1; sh <= 1; ++sh) {
+          hx = hx || clip_piece(sup.x, wx, X0, nX, A.Nx, sh * A.Nx, &i0, &l0) > 0;
+          hy = hy || clip_piece(sup.y, wy, Y0, nY, A.Ny, sh * A.Ny, &i0, &l0) > 0;
+        }
+        if (hx && hy) queue[atomicAdd(&qn, 1)] = 2 * p + kind;
+      }
+    }
+    __syncthreads();
+    // ---- spread the queued entries, one wavefront per entry ----
+    const int nq_entries = qn;
+    for (int qi = wave; qi < nq_entries; qi += nwaves) {
+      const int ent = queue[qi];
+      const int p = ent >> 1, kind = ent & 1;
+      const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
+      const KernelDesc& K = kind ? A.kd : A.km;
+      const int rs = kind ? A.rs_d : A.rs_m;
+      // the entry's weight record into this wave's LDS slot (one coalesced read)
+      const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * rs;
+      __builtin_amdgcn_wave_barrier();
+      for (int i = lane; i < rs; i += 64) st[i] = rec[i];
+      __builtin_amdgcn_wave_barrier();
+      const int W = K.wzmax, nz = sup.w;
+      const T* wxr = st;
+      const T* wyr = st + K.wx;
+      const T* dxr = wyr + K.wy;
+      const T* dyr = dxr + K.wx;
+      const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);
+      const T* rowr = zr + (kind ? 4 : 2) * W;
+      T f0, f1, f2;
+      if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
+      else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
+      for (int sxp = -1; sxp <= 1; ++sxp) {
+        int ix0, lx0;
+        const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
+        if (!cx) continue;
+        const float icx = 1.0f / (float)cx;
+        for (int syp = -1; syp <= 1; ++syp) {
+          int iy0, ly0;
+          const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
+          if (!cy) continue;
+          const int ncol = cx * cy;
+          const float incol = 1.0f / (float)ncol;
+          const int npts = ncol * nz;
+          for (int t = lane; t < npts; t += 64) {
+            // x fastest, then y, then z
+            int col, a;
+            const int q = fdiv(t, ncol, incol, &col);
+            const int b = fdiv(col, cx, icx, &a);
+            const int k = (int)rowr[q] - Z0;
+            if (k < 0 || k >= nZ) continue;
+            T* dst = acc + k * PL + (ly0 + b) * RS + (lx0 + a);
+            const int i = ix0 + a, j = iy0 + b;
+            if (kind == 0) {
+              const T w = wxr[i] * wyr[j] * zr[q];
+              atomicAdd(dst, w * f0);
+              atomicAdd(dst + vol, w * f1);
+              atomicAdd(dst + 2 * vol, w * f2);
+            } else {
+              const T wzA = zr[q], dzB = zr[W + q];
+              const T gx = dxr[i] * wyr[j] * wzA;
+              const T gy = wxr[i] * dyr[j] * wzA;
+              const T gz = wxr[i] * wyr[j] * dzB;
+              const T h = (T)0.5;
+              atomicAdd(dst, h * (gy * f2 - gz * f1));
+              atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
+              atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
             }
           }
         }
       }
-      __syncthreads();
-      if (threadIdx.x == 0) qn = 0;
-      __syncthreads();
     }
+    __syncthreads();
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
   }
-  // one write per grid element of the tile
-  const int nrows = 3 * nZ * nY;
-  for (int t = threadIdx.x; t < nrows * A.TX; t += blockDim.x) {
-    const int ix = t % A.TX;
-    const int row = t / A.TX;
-    if (ix >= nX) continue;
-    const int jy = row % nY;
-    const int r2 = row / nY;
-    const int kz = r2 % nZ;
-    const int c = r2 / nZ;
-    grid[(((size_t)c * A.Nz + Z0 + kz) * A.Ny + Y0 + jy) * A.Nx + X0 + ix] =
-        acc[c * vol + kz * plane + jy * A.TX + ix];
+  // one write per grid element of the tile: thread -> fixed x, rows (z, y)
+  // in steps of blockDim / TX (one float-reciprocal division per element)
+  {
+    const int ix = threadIdx.x % A.TX, rstep = blockDim.x / A.TX;
+    const int nrow = nZ * nY;
+    const float iny = 1.0f / (float)nY;
+    if (ix < nX && threadIdx.x < rstep * A.TX)
+      for (int c = 0; c < 3; ++c) {
+        T* gc = grid + ((size_t)c * A.Nz + Z0) * A.Ny * A.Nx + (size_t)Y0 * A.Nx + X0 + ix;
+        const T* ac = acc + c * vol + ix;
+        for (int row = threadIdx.x / A.TX; row < nrow; row += rstep) {
+          int jy;
+          const int kz = fdiv(row, nY, iny, &jy);
+          gc[((size_t)kz * A.Ny + jy) * A.Nx] = ac[kz * PL + jy * RS];
+        }
+      }
   }
 }
 
@@ -369,14 +398,16 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     const T* wzi = zr + (kind ? 2 : 1) * W;   // interpolation weights (value kernel)
     const T* dzi = zr + 3 * W;                // d/dz kernel (dipole only)
     const T* rowr = zr + (kind ? 4 : 2) * W;
-    const int npts = wx * wy * nz;
-    const float inz = 1.0f / (float)nz, iwx = 1.0f / (float)wx;
+    const int ncol = wx * wy;
+    const int npts = ncol * nz;
+    const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
     T d[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = 0;
     for (int t = lane; t < npts; t += 64) {
-      int q, a;
-      const int col = fdiv(t, nz, inz, &q);
+      // x fastest (contiguous loads), then y, then z
+      int col, a;
+      const int q = fdiv(t, ncol, incol, &col);
       const int b = fdiv(col, wx, iwx, &a);
       int x = sup.x + a, y = sup.y + b;
       x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
@@ -789,10 +820,17 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
                        order, N);
 }
 
+size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz)
+{
+  // accumulator, per-wave record slots (4 waves), queue, tile table + counter
+  return ((size_t)3 * A.ZC * (A.TY * (A.TX + 1) + 1) + 4 * (size_t)A.rs_max) * tsz +
+         (2 * 256 + 2 * DPS_SPREAD_MAXT + 2) * sizeof(int);
+}
+
 template <class T, class Tin>
 void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s)
 {
-  const size_t lds = (size_t)3 * A.ZC * A.TY * A.TX * sizeof(T) + (2 * 256 + 4) * sizeof(int);
+  const size_t lds = spread_lds_bytes(A, sizeof(T));
   dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
   hipLaunchKernelGGL((k_spread<T, Tin>), grid, dim3(256), lds, s, A, (const Tin*)F,
                      (const Tin*)Tq);

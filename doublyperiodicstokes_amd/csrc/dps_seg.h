@@ -356,7 +356,7 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345), in
   //      batches of NRV components sharing each scan ----
   const double ieta = 1.0 / B.eta;
-  cplx wall_bot[3], wall_top[3];
+  cplx wall_bot[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, wall_top[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
   #pragma unroll
   for (int cb = 0; cb < 3; cb += NRV) {
     DPS_STAGE_FENCE();
@@ -444,10 +444,17 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
         if (r0 + q < Nz) fout(c, q, cu[q]);
     }
     seg_sum<LPM>(ln, red);
+    // stored through selects on constant indices (a runtime index would put
+    // the two arrays in private memory)
     #pragma unroll
     for (int b = 0; b < NRV; ++b) {
-      wall_top[vel_comp<NRV>(cb + b)] = mk(red[4 * b], red[4 * b + 1]);
-      wall_bot[vel_comp<NRV>(cb + b)] = mk(red[4 * b + 2], red[4 * b + 3]);
+      const int c = vel_comp<NRV>(cb + b);
+      const cplx t = mk(red[4 * b], red[4 * b + 1]), bo = mk(red[4 * b + 2], red[4 * b + 3]);
+      #pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        wall_top[i] = csel(c == i, t, wall_top[i]);
+        wall_bot[i] = csel(c == i, bo, wall_bot[i]);
+      }
     }
   }
 #undef C0

@@ -75,17 +75,17 @@ struct DevLanes {
   }
 };
 
-// coefficient rows of one mode in LDS, layout [c][row][g]
+// coefficient rows of one mode in LDS, layout [c][row][g], row stride GS
 template <class T>
 struct LdsCoef {
   const typename C2<T>::t* S;
-  int G, g, Nz, r0;
+  int GS, g, Nz, r0;
   bool conj;
   __device__ cplx operator()(int c, int q) const
   {
     const int row = r0 + q;
     if (row >= Nz) return mk(0, 0);
-    const typename C2<T>::t v = S[(c * Nz + row) * G + g];
+    const typename C2<T>::t v = S[(c * Nz + row) * GS + g];
     return conj ? mk((double)v.x, -(double)v.y) : mk((double)v.x, (double)v.y);
   }
 };
@@ -95,11 +95,11 @@ struct LdsCoef {
 template <class T>
 struct LdsStore {
   typename C2<T>::t* S;
-  int G, g, Nz, r0;
+  int GS, g, Nz, r0;
   bool avg;
   __device__ void operator()(int c, int q, cplx v)
   {
-    typename C2<T>::t& d = S[(c * Nz + r0 + q) * G + g];
+    typename C2<T>::t& d = S[(c * Nz + r0 + q) * GS + g];
     if (avg) {
       d.x = (T)(0.5 * ((double)d.x + v.re));
       d.y = (T)(0.5 * ((double)d.y - v.im));
@@ -135,7 +135,10 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int Nz = A.B.Nz, G = A.G, Nm = A.Nm, NH = A.NH;
   const int ncol = 3 * Nz * G;
-  V* S = reinterpret_cast<V*>(smem_raw);   // [c][row][g]: values, coefficients, values
+  // [c][row][g] with row stride G+1: the lanes of one mode (different rows)
+  // fall in different LDS banks
+  const int GS = G + 1;
+  V* S = reinterpret_cast<V*>(smem_raw);   // values, coefficients, values
   const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
   const int g = threadIdx.x / LPM;
   const int l = threadIdx.x % LPM;
@@ -156,7 +159,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     V v;
     v.x = v.y = (T)0;
     if (m0 + gi < Nm) v = in[(size_t)row * Nm + m0 + gi];
-    S[i] = v;
+    S[row * GS + gi] = v;
   }
   __syncthreads();
 
@@ -174,8 +177,8 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
       const bool mid = (j == Nz - 1 - j);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const V a = S[(c * Nz + j) * G + g];
-        const V b = S[(c * Nz + Nz - 1 - j) * G + g];
+        const V a = S[(c * Nz + j) * GS + g];
+        const V b = S[(c * Nz + Nz - 1 - j) * GS + g];
         if (mid) { er[c] = a.x; ei[c] = a.y; orr[c] = oi[c] = (T)0; }
         else { er[c] = a.x + b.x; ei[c] = a.y + b.y; orr[c] = a.x - b.x; oi[c] = a.y - b.y; }
       }
@@ -199,7 +202,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
           V v;
           v.x = ar[c][q];
           v.y = ai[c][q];
-          S[(c * Nz + r0 + q) * G + g] = v;
+          S[(c * Nz + r0 + q) * GS + g] = v;
           if (special) out[(size_t)(c * Nz + r0 + q) * Nm + m] = v;
         }
   }
@@ -214,8 +217,8 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   double K = 0;
   if (m < Nm) {
     const DevLanes<LPM> ln{l};
-    LdsCoef<T> fin{S, G, g, Nz, r0, false};
-    LdsStore<T> fout{S, G, g, Nz, r0, false};
+    LdsCoef<T> fin{S, GS, g, Nz, r0, false};
+    LdsStore<T> fout{S, GS, g, Nz, r0, false};
     if (m == 0) {
       seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);
     } else {
@@ -274,7 +277,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     for (int k = 0; k < Nz; k += 2) {
       V u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * G + g];
+      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * GS + g];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const T t = Ti[((size_t)k * R + r) * LPM + l];
@@ -289,7 +292,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     for (int k = 1; k < Nz; k += 2) {
       V u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * G + g];
+      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * GS + g];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const T t = Ti[((size_t)k * R + r) * LPM + l];
@@ -337,7 +340,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
             v.x += (T)(sr * A.inv_nxy);
             v.y += (T)(si * A.inv_nxy);
           }
-          S[(c * Nz + jj) * G + g] = v;
+          S[(c * Nz + jj) * GS + g] = v;
         }
       }
     }
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
   // ---- phase 4: columns out ----
   for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
     const int gi = i % G, row = i / G;
-    if (m0 + gi < Nm) out[(size_t)row * Nm + m0 + gi] = S[i];
+    if (m0 + gi < Nm) out[(size_t)row * Nm + m0 + gi] = S[row * GS + gi];
   }
 }
 
@@ -394,9 +397,9 @@ bool spectral_shape(int Nz, int* lpm, int* e)
 
 int spectral_modes_per_block(int Nz, int lpm, bool mixed)
 {
-  const size_t per_mode = (size_t)3 * Nz * (mixed ? sizeof(float2) : sizeof(double2));
+  const size_t per_row = (size_t)3 * Nz * (mixed ? sizeof(float2) : sizeof(double2));
   int G = 256 / lpm;
-  while (G > 1 && (size_t)G * per_mode > 65536) G /= 2;
+  while (G > 1 && (size_t)(G + 1) * per_row > 65536) G /= 2;
   return G;
 }
 
@@ -412,7 +415,7 @@ template <class T>
 void launch_spectral(const SpecArgs& A, const void* in, void* out, hipStream_t s)
 {
   typedef typename C2<T>::t V;
-  const size_t lds = (size_t)3 * A.B.Nz * A.G * sizeof(V);
+  const size_t lds = (size_t)3 * A.B.Nz * (A.G + 1) * sizeof(V);
   const dim3 grid((A.Nm + A.G - 1) / A.G), block(A.G * A.lpm);
   const V* i = (const V*)in;
   V* o = (V*)out;
