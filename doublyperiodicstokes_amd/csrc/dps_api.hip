@@ -212,8 +212,19 @@ int build(dps_handle* h)
   h->ct = build_cheb_transform(h->Nz);
   h->fused = (P.solver == DPS_SOLVER_AUTO) && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
   if (h->fused) {
-    h->sp_G = spectral_modes_per_block(h->Nz, h->sp_lpm, h->prec == DPS_PREC_MIXED);
-  } else {
+    // up to 64 KiB per workgroup keeps several workgroups per CU; beyond
+    // that, what the device allows, else the unfused chain
+    const bool mx = h->prec == DPS_PREC_MIXED;
+    h->sp_G = spectral_modes_per_block(h->sp_lpm, h->sp_e, mx);
+    const size_t lds = spectral_lds_bytes(h->sp_lpm, h->sp_e, h->sp_G, mx);
+    if (lds > 65536) {
+      int dev = 0, lds_max = 0;
+      h->fused = hipGetDevice(&dev) == hipSuccess &&
+                 hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+                 lds <= (size_t)lds_max;
+    }
+  }
+  if (!h->fused) {
     ZArgs Z;
     Z.N = h->Nz;
     Z.NH = h->ct.NH;
@@ -367,24 +378,38 @@ int allocate_fused(dps_handle* h)
 {
   int rc;
   const int Nz = h->Nz, NH = h->ct.NH, LPM = h->sp_lpm, E = h->sp_e, R = E / 2;
+  const int NP = (Nz + 1) / 2;   // coefficient pairs (2p, 2p+1)
   const double inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
-  std::vector<double> tf((size_t)NH * E * LPM, 0.0), ti((size_t)Nz * R * LPM, 0.0);
+  // forward [NH][LPM][E]: coefficient l E + q of node pair j
+  // inverse [NP][LPM][E]: node pair l + LPM r of coefficient 2p (q = r) and
+  // of coefficient 2p+1 (q = R + r)
+  std::vector<double> tf((size_t)NH * LPM * E, 0.0), ti((size_t)NP * LPM * E, 0.0);
   for (int j = 0; j < NH; ++j)
-    for (int q = 0; q < E; ++q)
-      for (int l = 0; l < LPM; ++l) {
+    for (int l = 0; l < LPM; ++l)
+      for (int q = 0; q < E; ++q) {
         const int k = l * E + q;
         if (k >= Nz) continue;
-        tf[((size_t)j * E + q) * LPM + l] =
+        tf[((size_t)j * LPM + l) * E + q] =
             (k & 1) ? h->ct.TfO[(size_t)(k / 2) * NH + j] : h->ct.TfE[(size_t)(k / 2) * NH + j];
       }
   for (int k = 0; k < Nz; ++k)
-    for (int r = 0; r < R; ++r)
-      for (int l = 0; l < LPM; ++l) {
+    for (int l = 0; l < LPM; ++l)
+      for (int r = 0; r < R; ++r) {
         const int j = l + LPM * r;
         if (j >= NH) continue;
-        ti[((size_t)k * R + r) * LPM + l] =
+        ti[((size_t)(k / 2) * LPM + l) * E + (k & 1) * R + r] =
             inv_nxy * ((k & 1) ? h->ct.TiO[(size_t)j * h->ct.NO + k / 2] : h->ct.TiE[(size_t)j * h->ct.NE + k / 2]);
       }
+  // k = 0 and the modes on a Nyquist index, solved by the special kernel
+  std::vector<int> sp;
+  for (int iy = 0; iy < h->Ny; ++iy)
+    for (int ix = 0; ix < h->Nkx; ++ix) {
+      const bool s = (ix == 0 && iy == 0) || ((h->Nx % 2 == 0) && ix == h->Nx / 2) ||
+                     ((h->Ny % 2 == 0) && iy == h->Ny / 2);
+      if (s) sp.push_back(iy * h->Nkx + ix);
+    }
+  h->nspecial = (int)sp.size();
+  if ((rc = h->upload(&h->d_special, sp))) return rc;
   if (h->prec == DPS_PREC_MIXED) {
     if ((rc = h->upload((float**)&h->d_TfP, cast_vec<float>(tf)))) return rc;
     if ((rc = h->upload((float**)&h->d_TiP, cast_vec<float>(ti)))) return rc;
@@ -396,7 +421,6 @@ int allocate_fused(dps_handle* h)
   if ((rc = h->alloc(&h->d_fib, nf))) return rc;
   if ((rc = h->alloc(&h->d_fab, nf))) return rc;
   if ((rc = h->alloc(&h->d_fG, (size_t)h->Nm * 4))) return rc;
-  if ((rc = h->alloc(&h->d_corr, (size_t)h->Nm * 12))) return rc;
   double *t_ib = nullptr, *t_ab = nullptr;
   if (hipMalloc((void**)&t_ib, (size_t)(Nz + 1) * h->Nm * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&t_ab, (size_t)Nz * h->Nm * sizeof(double)) != hipSuccess) {
@@ -564,7 +588,8 @@ SpecArgs spec_args(dps_handle* h)
   A.lpm = h->sp_lpm; A.e = h->sp_e; A.G = h->sp_G;
   A.TfP = h->d_TfP; A.TiP = h->d_TiP;
   A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
-  A.cstash = h->d_corr;
+  A.kxs = h->d_kxs; A.kys = h->d_kys;
+  A.special = h->d_special; A.nspecial = h->nspecial;
   A.zpts = h->d_zpts;
   A.Hs = h->Lz;
   A.inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);

@@ -118,18 +118,40 @@ struct SpecArgs {
   double Lx, Ly;
   int NH;                 // node pairs ceil(Nz/2)
   int lpm, e, G;          // lanes per mode, rows per lane, modes per block
-  const void* TfP;        // [NH][E][LPM] forward rows in lane order (grid precision)
-  const void* TiP;        // [Nz][E/2][LPM] inverse rows in lane order (scaled by 1/(Nx Ny))
-  const double* fac_ib;   // [Nm][E][LPM] reciprocal pivots of each lane's rows
-  const double* fac_ab;   // [Nm][E][LPM] A^{-1}B of each lane's rows
+  const void* TfP;        // [NH][LPM][E] forward rows in lane order (grid precision)
+  const void* TiP;        // [ceil(Nz/2)][LPM][E] inverse rows in lane order (scaled by 1/(Nx Ny))
+  const double* fac_ib;   // [Nm][LPM][E] reciprocal pivots of each lane's rows
+  const double* fac_ab;   // [Nm][LPM][E] A^{-1}B of each lane's rows
   const double* fac_G;    // [Nm][4] inverse Schur complement
-  double2* cstash;        // [Nm][12] first-solve corrections of Nyquist-touching modes
+  const double *kxs, *kys;   // grid wavenumbers by index
+  const int* special;     // k = 0 and the Nyquist-touching modes
+  int nspecial;
   const double* zpts;
   double Hs, inv_nxy;
 };
 // lane-group shape for Nz rows: returns false when Nz is too large
 bool spectral_shape(int Nz, int* lpm, int* e);
-int spectral_modes_per_block(int Nz, int lpm, bool mixed);
+// LDS of one spectral workgroup: 9 row tables of LPM E doubles, 12 wall
+// corrections per mode, 3 components x LPM E rows x (G + 1) columns
+constexpr size_t spectral_lds_bytes(int lpm, int e, int G, bool mixed)
+{
+  return (size_t)9 * lpm * e * 8 + (size_t)G * 12 * 16 + (size_t)3 * lpm * e * (G + 1) * (mixed ? 8 : 16);
+}
+// modes per workgroup: the most (up to 256 lanes) within 64 KiB of LDS, else 1
+constexpr int spectral_modes_per_block(int lpm, int e, bool mixed)
+{
+  int G = 256 / lpm;
+  while (G > 1 && spectral_lds_bytes(lpm, e, G, mixed) > 65536) G /= 2;
+  return G;
+}
+// waves per SIMD that the LDS of one CU (160 KiB) allows
+constexpr int spectral_waves_per_simd(int lpm, int e, bool mixed)
+{
+  const int G = spectral_modes_per_block(lpm, e, mixed);
+  const int blocks = (int)(163840 / spectral_lds_bytes(lpm, e, G, mixed));
+  const int w = blocks * (G * lpm / 64) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
 void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
                        double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,
                        hipStream_t s);

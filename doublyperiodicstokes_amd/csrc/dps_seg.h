@@ -13,17 +13,27 @@
 //   * five-point stencils (SIMat / FIMat products): two halo rows from each
 //     neighbouring lane.
 // Lane exchange goes through a policy object L with members
-//   int l;  template <int D> double up(v), down(v), xr(v);
-// up/down: the value of lane l-D / l+D of the group (any value when outside);
-// xr: the value of lane l's partner in butterfly stage D (an involution that
-// pairs the two halves of each aligned 2D-lane block, e.g. l ^ D).  The gfx950
-// kernel binds it to DPP row operations (16-lane groups) or lane permutes;
-// the host-side checks bind it to threads joined by a barrier, so the
-// identical code runs on both.
+//   int l;  template <int D> double up(v, fill), down(v, fill), xr(v);
+// up/down: the value of lane l-D / l+D of the group, or `fill` where that
+// lane is outside the group; xr: the value of lane l's partner in butterfly
+// stage D (an involution that pairs the two halves of each aligned 2D-lane
+// block, e.g. l ^ D).  Fills are chosen so that every lane runs the same
+// instructions (identity maps in scans, zero halos): no lane-dependent
+// branches.  The gfx950 kernel binds the policy to DPP row operations (16-lane
+// groups, the fill entering as the DPP `old` operand) or lane permutes; the
+// host-side checks bind it to threads joined by a barrier, so the identical
+// code runs on both.
 //
 // Rows >= Nz are padding: their inputs are zero, their factors are zero, and
 // the rows Nz and Nz+1 carry the two Schur unknowns where a stencil reads
 // them, as the sequential solver's XP(Nz) = Y0, XP(Nz+1) = Y1.
+//
+// Row tables.  seg_bvp_mode reads the row tables of B (sd0, sm2, sp2, fm1,
+// fp1, R1, R2, L1, L2) at every row in [0, LPM*E) without bounds tests, so
+// they must be readable there and zero at rows >= Nz (build_cheb_tables
+// already has sm2 = 0 at rows 0, 1 and fm1 = 0 at row 0).  fin(c, q) returns
+// zero for rows >= Nz; fout(c, q, v) is called for every row and must drop or
+// absorb the rows >= Nz.  seg_bvp_k0 takes the unpadded tables.
 #pragma once
 #include "dps_core.h"
 
@@ -47,15 +57,17 @@ DPS_HD void seg_sum(const L& ln, double (&v)[N])
 }
 
 template <bool UP, int D, class L>
-DPS_HD double shift(const L& ln, double v)
+DPS_HD double shift(const L& ln, double v, double fill)
 {
-  if constexpr (UP) return ln.template up<D>(v);
-  else return ln.template down<D>(v);
+  if constexpr (UP) return ln.template up<D>(v, fill);
+  else return ln.template down<D>(v, fill);
 }
 
 // Inclusive scan over the group of the affine maps y -> a + b y held by each
 // lane (UP: lane 0 applies first; !UP: lane LPM-1 applies first).  NC chains
-// with their own multipliers, NR right-hand sides per chain.
+// with their own multipliers, NR right-hand sides per chain.  A lane with no
+// partner at stage D receives the identity map (a = 0, b = 1), which leaves
+// its own map unchanged.
 template <int LPM, int NC, int NR, bool UP, int D = 1, class L>
 DPS_HD void seg_scan(const L& ln, cplx (&a)[NC][NR], double (&b)[NC])
 {
@@ -64,18 +76,17 @@ DPS_HD void seg_scan(const L& ln, cplx (&a)[NC][NR], double (&b)[NC])
     cplx as[NC][NR];
     #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      bs[c] = shift<UP, D>(ln, b[c]);
+      bs[c] = shift<UP, D>(ln, b[c], 1.0);
       #pragma unroll
-      for (int r = 0; r < NR; ++r) as[c][r] = mk(shift<UP, D>(ln, a[c][r].re), shift<UP, D>(ln, a[c][r].im));
+      for (int r = 0; r < NR; ++r)
+        as[c][r] = mk(shift<UP, D>(ln, a[c][r].re, 0.0), shift<UP, D>(ln, a[c][r].im, 0.0));
     }
-    const bool take = UP ? (ln.l >= D) : (ln.l + D < LPM);
-    if (take)
+    #pragma unroll
+    for (int c = 0; c < NC; ++c) {
       #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        #pragma unroll
-        for (int r = 0; r < NR; ++r) a[c][r] = a[c][r] + b[c] * as[c][r];
-        b[c] = b[c] * bs[c];
-      }
+      for (int r = 0; r < NR; ++r) a[c][r] = a[c][r] + b[c] * as[c][r];
+      b[c] = b[c] * bs[c];
+    }
     seg_scan<LPM, NC, NR, UP, 2 * D>(ln, a, b);
   }
 }
@@ -85,37 +96,37 @@ DPS_HD void seg_scan(const L& ln, cplx (&a)[NC][NR], double (&b)[NC])
 template <int LPM, int NC, int NR, bool UP, class L>
 DPS_HD void seg_carry(const L& ln, const cplx (&a)[NC][NR], cplx (&cin)[NC][NR])
 {
-  const bool first = UP ? (ln.l == 0) : (ln.l == LPM - 1);
   #pragma unroll
   for (int c = 0; c < NC; ++c)
     #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const cplx v = mk(shift<UP, 1>(ln, a[c][r].re), shift<UP, 1>(ln, a[c][r].im));
-      cin[c][r] = first ? mk(0, 0) : v;
-    }
+    for (int r = 0; r < NR; ++r)
+      cin[c][r] = mk(shift<UP, 1>(ln, a[c][r].re, 0.0), shift<UP, 1>(ln, a[c][r].im, 0.0));
 }
 
 // Sweep multipliers of one lane's rows, formed on demand from the reciprocal
 // pivots (so they need no registers of their own):
 //   lz(q) = DM2(row-2) / pivot(row-2)   (forward elimination)
 //   bw(q) = DP2(row) / pivot(row)       (back substitution)
+// sm2 is zero at rows 0, 1 and >= Nz, so lz needs no row test; bw is zero
+// from row Nz-2 on, where sp2 still feeds the S band.
 template <int E>
 struct SegMul {
   const BvpConsts* B;
   double Ksq;
   int Nz, r0;
   const double (*ib)[E];
-  double ibm0, ibm1;   // reciprocal pivots of rows r0-2, r0-1 (lane below)
+  double ibm0, ibm1;   // reciprocal pivots of rows r0-2, r0-1 (lane below; 0 on lane 0)
   DPS_HD double lz(int q) const
   {
     const int row = r0 + q;
     const double ibprev = (q >= 2) ? (*ib)[q - 2] : (q == 0 ? ibm0 : ibm1);
-    return (row >= 2 && row < Nz) ? (0.0 - B->sm2[row] * Ksq) * ibprev : 0.0;
+    return (0.0 - B->sm2[row] * Ksq) * ibprev;
   }
   DPS_HD double bw(int q) const
   {
     const int row = r0 + q;
-    return (row < Nz - 2) ? (0.0 - B->sp2[row] * Ksq) * (*ib)[q] : 0.0;
+    const double s = B->sp2[row];
+    return (0.0 - (row < Nz - 2 ? s : 0.0) * Ksq) * (*ib)[q];
   }
 };
 
@@ -195,15 +206,10 @@ DPS_HD void seg_halo(const L& ln, const cplx (&X)[E], int Nz, cplx Y0, cplx Y1, 
 {
   #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const cplx u = mk(ln.template up<1>(X[E - 2 + i].re), ln.template up<1>(X[E - 2 + i].im));
-    const cplx v = mk(ln.template down<1>(X[i].re), ln.template down<1>(X[i].im));
-    lo[i] = (ln.l == 0) ? mk(0, 0) : u;
-    if (ln.l == LPM - 1) {
-      const int row = LPM * E + i;
-      hi[i] = (row == Nz) ? Y0 : ((row == Nz + 1) ? Y1 : mk(0, 0));
-    } else {
-      hi[i] = v;
-    }
+    const int row = LPM * E + i;   // beyond the last lane
+    const cplx past = (row == Nz) ? Y0 : ((row == Nz + 1) ? Y1 : mk(0, 0));
+    lo[i] = mk(ln.template up<1>(X[E - 2 + i].re, 0.0), ln.template up<1>(X[E - 2 + i].im, 0.0));
+    hi[i] = mk(ln.template down<1>(X[i].re, past.re), ln.template down<1>(X[i].im, past.im));
   }
 }
 
@@ -224,13 +230,26 @@ DPS_HD cplx s_band(const BvpConsts& B, int row, cplx xm2, cplx x0, cplx x2, cplx
   return v;
 }
 
-// Rows of the Schur solution x - A^{-1}B Y, extended by Y0, Y1 at rows Nz, Nz+1.
-DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y0, cplx Y1)
+// S band of a lane's row q (row = l*E + q): the row-0 and row-1 couplings to
+// the Schur unknowns belong to lane 0, q = 0 and 1.  Same association order
+// as s_band; the table terms vanish on padding rows.
+DPS_HD cplx seg_sband(const BvpConsts& B, int row, int q, bool lane0, cplx xm2, cplx x0, cplx x2, cplx Y0,
+                      cplx Y1)
 {
-  if (row < Nz) return x - ((row & 1) ? ab * Y1 : ab * Y0);
-  if (row == Nz) return Y0;
-  if (row == Nz + 1) return Y1;
-  return mk(0, 0);
+  cplx v = B.sd0[row] * x0;
+  v = B.sm2[row] * xm2 + v;   // sm2 = 0 at rows 0, 1
+  v = v + B.sp2[row] * x2;
+  if (q == 0) v = v + (lane0 ? B.S0N : 0.0) * Y0;
+  if (q == 1) v = v + (lane0 ? B.S1N1 : 0.0) * Y1;
+  return v;
+}
+
+// Rows of the Schur solution x - A^{-1}B Y, extended by Y0, Y1 at rows Nz, Nz+1.
+// (x and ab are zero at rows >= Nz, and E is even, so row parity is q parity.)
+DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y, cplx Y0, cplx Y1)
+{
+  const cplx v = x - ab * Y;
+  return csel(row == Nz, Y0, csel(row == Nz + 1, Y1, v));
 }
 
 // One mode k != 0 (counterpart of bvp_mode); NRV (1 or 3) velocity components
@@ -238,9 +257,14 @@ DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y0, cplx Y1)
 // static factorisation (ib = 1/pivot of the row, i.e. IB(row+1)); Gi: inverse
 // Schur complement.  fin(c, q) returns the forcing coefficient of component c
 // at row l*E+q (zero for rows >= Nz); fout(c, q, v) receives the velocity
-// coefficients of the rows < Nz; res gets the wall correction (identical on
-// every lane of the group).
-template <int LPM, int E, int NRV = 3, class L, class FIn, class FOut>
+// coefficients; res gets the wall correction (identical on every lane of the
+// group).
+// STAGE: the x and y velocity right-hand sides (D_c p - f_c) / eta are formed
+// as soon as the pressure is known and written through fout over the x and y
+// forcing, then read back through fin by their solves, so the pressure
+// coefficients hold no registers during the velocity solves.  fin and fout
+// must then address the same storage, and fout must store what it is given.
+template <int LPM, int E, int NRV = 3, bool STAGE = false, class L, class FIn, class FOut>
 DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, int nyq_x, int nyq_y,
                          const double (&ib)[E], const double (&ab)[E], const double* Gi, const FIn& fin,
                          FOut& fout, ModeResult& res)
@@ -264,8 +288,9 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   mu.Nz = Nz;
   mu.r0 = r0;
   mu.ib = &ib;
-  mu.ibm0 = ln.template up<1>(ib[E - 2]);
-  mu.ibm1 = ln.template up<1>(ib[E - 1]);
+  mu.ibm0 = ln.template up<1>(ib[E - 2], 0.0);
+  mu.ibm1 = ln.template up<1>(ib[E - 1], 0.0);
+  const bool lane0 = (ln.l == 0);
 
   // ---- pressure rhs: Dx fx + Dy fy + chebCoeffDiff(fz) (DPTools.cpp:134-195) ----
   cplx X[1][E];
@@ -275,7 +300,7 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     #pragma unroll
     for (int q = E - 1; q >= 0; --q) {
       const int row = r0 + q;
-      const cplx w = (row < Nz) ? ((double)(row * 2) * iH) * fin(2, q) : mk(0, 0);
+      const cplx w = ((double)(row * 2) * iH) * fin(2, q);
       s[q & 1][0] = s[q & 1][0] + w;
       suf[q] = s[q & 1][0];
     }
@@ -285,10 +310,9 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     seg_carry<LPM, 2, 1, false>(ln, s, above);
     #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int row = r0 + q;
       cplx df = (q + 1 < E) ? suf[q + 1] + above[(q + 1) & 1][0] : above[0][0];
-      if (row == 0) df = df / 2;
-      X[0][q] = (row < Nz) ? Dx * fin(0, q) + Dy * fin(1, q) + df : mk(0, 0);
+      if (q == 0) df = df * (lane0 ? 0.5 : 1.0);   // row 0
+      X[0][q] = Dx * fin(0, q) + Dy * fin(1, q) + df;
     }
   }
   DPS_STAGE_FENCE();
@@ -302,11 +326,9 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int row = r0 + q;
-      if (row < Nz) {
-        const double c0 = C0(row), c1 = C1(row);
-        y[0] += X[0][q].re * c0; y[1] += X[0][q].im * c0;
-        y[2] += X[0][q].re * c1; y[3] += X[0][q].im * c1;
-      }
+      const double c0 = C0(row), c1 = C1(row);
+      y[0] += X[0][q].re * c0; y[1] += X[0][q].im * c0;
+      y[2] += X[0][q].re * c1; y[3] += X[0][q].im * c1;
     }
     seg_sum<LPM>(ln, y);
     const cplx y0 = mk(y[0], y[1]), y1 = mk(y[2], y[3]);
@@ -315,7 +337,7 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   }
   // x = xp - A^{-1}B Y ; Cp = SIMat x ; Dp = FIMat x  (DPTools.cpp:266-284)
   #pragma unroll
-  for (int q = 0; q < E; ++q) X[0][q] = seg_xrow(r0 + q, Nz, X[0][q], ab[q], Y0, Y1);
+  for (int q = 0; q < E; ++q) X[0][q] = seg_xrow(r0 + q, Nz, X[0][q], ab[q], (q & 1) ? Y1 : Y0, Y0, Y1);
   cplx Cp[E], Dp[E];
   cplx psum, palt;
   {
@@ -325,27 +347,31 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int row = r0 + q;
-      if (row < Nz) {
-        const cplx xm2 = (q >= 2) ? X[0][q - 2] : lo[q];
-        const cplx xm1 = (q >= 1) ? X[0][q - 1] : lo[1];
-        const cplx x1 = (q + 1 < E) ? X[0][q + 1] : hi[q + 1 - E];
-        const cplx x2 = (q + 2 < E) ? X[0][q + 2] : hi[q + 2 - E];
-        Cp[q] = s_band(B, row, xm2, X[0][q], x2, Y0, Y1);
-        cplx dp = B.fp1[row] * x1;
-        if (row >= 1) dp = B.fm1[row] * xm1 + dp;
-        if (row == 0) dp = dp + B.F0N1 * Y1;
-        Dp[q] = dp;
-        red[0] += Cp[q].re; red[1] += Cp[q].im;
-        const double sg = (row & 1) ? -1.0 : 1.0;
-        red[2] += sg * Cp[q].re; red[3] += sg * Cp[q].im;
-      } else {
-        Cp[q] = mk(0, 0);
-        Dp[q] = mk(0, 0);
-      }
+      const cplx xm2 = (q >= 2) ? X[0][q - 2] : lo[q];
+      const cplx xm1 = (q >= 1) ? X[0][q - 1] : lo[1];
+      const cplx x1 = (q + 1 < E) ? X[0][q + 1] : hi[q + 1 - E];
+      const cplx x2 = (q + 2 < E) ? X[0][q + 2] : hi[q + 2 - E];
+      Cp[q] = seg_sband(B, row, q, lane0, xm2, X[0][q], x2, Y0, Y1);
+      cplx dp = B.fp1[row] * x1;
+      dp = B.fm1[row] * xm1 + dp;   // fm1 = 0 at row 0
+      if (q == 0) dp = dp + (lane0 ? B.F0N1 : 0.0) * Y1;
+      Dp[q] = dp;
+      red[0] += Cp[q].re; red[1] += Cp[q].im;
+      const double sg = (q & 1) ? -1.0 : 1.0;
+      red[2] += sg * Cp[q].re; red[3] += sg * Cp[q].im;
     }
     seg_sum<LPM>(ln, red);
     psum = mk(red[0], red[1]);
     palt = mk(red[2], red[3]);
+  }
+  const double ieta = 1.0 / B.eta;
+  if constexpr (STAGE) {
+    DPS_STAGE_FENCE();
+    #pragma unroll
+    for (int q = 0; q < E; ++q) {
+      fout(0, q, (Dx * Cp[q] - fin(0, q)) * ieta);
+      fout(1, q, (Dy * Cp[q] - fin(1, q)) * ieta);
+    }
   }
   // BC rows for the velocity solves (DPTools.cpp:218-234), formed where used:
   //   bc[2c] = -D_c psum / (2 eta K), bc[2c+1] = D_c palt / (2 eta K)  (c = 0, 1)
@@ -355,7 +381,6 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
 
   // ---- velocity: three Schur solves (DPTools.cpp:197-216, 287-345), in
   //      batches of NRV components sharing each scan ----
-  const double ieta = 1.0 / B.eta;
   cplx wall_bot[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, wall_top[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
   #pragma unroll
   for (int cb = 0; cb < 3; cb += NRV) {
@@ -363,15 +388,14 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     cplx U[NRV][E];
     #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int row = r0 + q;
       #pragma unroll
       for (int b = 0; b < NRV; ++b) {
         const int c = vel_comp<NRV>(cb + b);
-        if (row < Nz) {
+        if (STAGE && c != 2) {
+          U[b][q] = fin(c, q);
+        } else {
           const cplx g = (c == 0) ? Dx * Cp[q] : ((c == 1) ? Dy * Cp[q] : Dp[q]);
           U[b][q] = (g - fin(c, q)) * ieta;
-        } else {
-          U[b][q] = mk(0, 0);
         }
       }
     }
@@ -388,13 +412,11 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
       #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int row = r0 + q;
-        if (row < Nz) {
-          const double c0 = C0(row), c1 = C1(row);
-          #pragma unroll
-          for (int b = 0; b < NRV; ++b) {
-            y[4 * b + 0] += U[b][q].re * c0; y[4 * b + 1] += U[b][q].im * c0;
-            y[4 * b + 2] += U[b][q].re * c1; y[4 * b + 3] += U[b][q].im * c1;
-          }
+        const double c0 = C0(row), c1 = C1(row);
+        #pragma unroll
+        for (int b = 0; b < NRV; ++b) {
+          y[4 * b + 0] += U[b][q].re * c0; y[4 * b + 1] += U[b][q].im * c0;
+          y[4 * b + 2] += U[b][q].re * c1; y[4 * b + 3] += U[b][q].im * c1;
         }
       }
       seg_sum<LPM>(ln, y);
@@ -423,25 +445,22 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
     for (int b = 0; b < NRV; ++b) {
       const int c = vel_comp<NRV>(cb + b);
       #pragma unroll
-      for (int q = 0; q < E; ++q) U[b][q] = seg_xrow(r0 + q, Nz, U[b][q], ab[q], Ya[b], Yb[b]);
+      for (int q = 0; q < E; ++q) U[b][q] = seg_xrow(r0 + q, Nz, U[b][q], ab[q], (q & 1) ? Yb[b] : Ya[b], Ya[b], Yb[b]);
       cplx lo[2], hi[2];
       seg_halo<LPM, E>(ln, U[b], Nz, Ya[b], Yb[b], lo, hi);
       cplx cu[E];
       #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int row = r0 + q;
-        if (row < Nz) {
-          const cplx xm2 = (q >= 2) ? U[b][q - 2] : lo[q];
-          const cplx x2 = (q + 2 < E) ? U[b][q + 2] : hi[q + 2 - E];
-          cu[q] = s_band(B, row, xm2, U[b][q], x2, Ya[b], Yb[b]);
-          red[4 * b + 0] += cu[q].re; red[4 * b + 1] += cu[q].im;
-          const double sg = (row & 1) ? -1.0 : 1.0;
-          red[4 * b + 2] += sg * cu[q].re; red[4 * b + 3] += sg * cu[q].im;
-        }
+        const cplx xm2 = (q >= 2) ? U[b][q - 2] : lo[q];
+        const cplx x2 = (q + 2 < E) ? U[b][q + 2] : hi[q + 2 - E];
+        cu[q] = seg_sband(B, row, q, lane0, xm2, U[b][q], x2, Ya[b], Yb[b]);
+        red[4 * b + 0] += cu[q].re; red[4 * b + 1] += cu[q].im;
+        const double sg = (q & 1) ? -1.0 : 1.0;
+        red[4 * b + 2] += sg * cu[q].re; red[4 * b + 3] += sg * cu[q].im;
       }
       #pragma unroll
-      for (int q = 0; q < E; ++q)
-        if (r0 + q < Nz) fout(c, q, cu[q]);
+      for (int q = 0; q < E; ++q) fout(c, q, cu[q]);
     }
     seg_sum<LPM>(ln, red);
     // stored through selects on constant indices (a runtime index would put

@@ -6,20 +6,28 @@
 // Fourier modes; each mode is solved by a group of LPM lanes of one wavefront,
 // lane l owning the E consecutive Chebyshev rows [l E, l E + E) (dps_seg.h).
 //
-//   phase 0  the G columns of the r2c spectrum, all z levels and components,
-//            are read once from HBM (rows of G consecutive modes) into LDS
+//   phase 0  the row tables of the solve are copied to LDS (zero past Nz), and
+//            the G columns of the r2c spectrum, all z levels and components,
+//            are read once from HBM (rows of G modes) into LDS
 //   phase 1  forward transform: lane l forms the coefficients of its rows
 //            from even/odd folded node pairs (grid precision), written back
 //            in place
-//   phase 2  FP64 BVP on the lane group, in place; modes touching a Nyquist
-//            index are solved twice (mode and mirror) and averaged, the
-//            mirror reading a copy of the forcing kept in its output column
+//   phase 2  FP64 BVP on the lane group, in place
 //   phase 3  inverse transform of node pairs (l + LPM r, Nz-1-l-LPM r) plus
 //            the analytic wall profiles, written back in place
-//   phase 4  the G columns are written back as rows of G consecutive modes
+//   phase 4  the G columns are written back as rows of G modes
 //
 // The spectrum crosses HBM once in each direction; everything in between
-// stays in one LDS buffer of 3 Nz G elements and registers.
+// stays in LDS and registers.
+//
+// Two instantiations share this code.  The regular kernel takes G consecutive
+// modes per workgroup and solves every mode except k = 0 and the modes on a
+// Nyquist index, whose columns it leaves untouched.  The special kernel takes
+// those from a list: k = 0 gets its own solve, and a Nyquist-touching mode is
+// solved twice -- as itself and as its mirror on the conjugated forcing, kept
+// meanwhile in the mode's own output column -- and averaged, because the
+// reference's c2c transform keeps only the real part of the inverse.  The
+// regular kernel thus carries no code or registers for the rare cases.
 #include <hip/hip_runtime.h>
 #include "dps_internal.h"
 #include "dps_seg.h"
@@ -31,14 +39,32 @@ namespace dps {
 
 namespace {
 
+// row tables of the solve held in LDS, rows [0, LPM E), zero from row Nz on
+enum { TB_SD0, TB_SM2, TB_SP2, TB_FM1, TB_FP1, TB_R1, TB_R2, TB_L1, TB_L2, TB_N };
+
+__device__ __forceinline__ const double* row_table(const BvpConsts& B, int t)
+{
+  switch (t) {
+    case TB_SD0: return B.sd0;
+    case TB_SM2: return B.sm2;
+    case TB_SP2: return B.sp2;
+    case TB_FM1: return B.fm1;
+    case TB_FP1: return B.fp1;
+    case TB_R1: return B.R1;
+    case TB_R2: return B.R2;
+    case TB_L1: return B.L1;
+    default: return B.L2;
+  }
+}
+
 // 64-bit DPP move within a 16-lane row; lanes whose source is outside the row
-// keep their own value
+// receive `fill` (the DPP old operand with bound_ctrl off)
 template <int CTRL>
-__device__ __forceinline__ double dpp64(double v)
+__device__ __forceinline__ double dpp64(double v, double fill)
 {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  lo = __builtin_amdgcn_update_dpp(__double2loint(fill), lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), hi, CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 
@@ -50,56 +76,61 @@ template <int LPM>
 struct DevLanes {
   int l;
   template <int D>
-  __device__ double up(double v) const
+  __device__ double up(double v, double fill) const
   {
-    if constexpr (LPM == 16) return dpp64<0x110 + D>(v);   // row_shr:D
-    else return __shfl_up(v, (unsigned)D, LPM);
+    if constexpr (LPM == 16) return dpp64<0x110 + D>(v, fill);   // row_shr:D
+    else {
+      const double r = __shfl_up(v, (unsigned)D, LPM);
+      return (l >= D) ? r : fill;
+    }
   }
   template <int D>
-  __device__ double down(double v) const
+  __device__ double down(double v, double fill) const
   {
-    if constexpr (LPM == 16) return dpp64<0x100 + D>(v);   // row_shl:D
-    else return __shfl_down(v, (unsigned)D, LPM);
+    if constexpr (LPM == 16) return dpp64<0x100 + D>(v, fill);   // row_shl:D
+    else {
+      const double r = __shfl_down(v, (unsigned)D, LPM);
+      return (l + D < LPM) ? r : fill;
+    }
   }
   template <int D>
   __device__ double xr(double v) const
   {
     if constexpr (LPM == 16) {
-      if constexpr (D == 1) return dpp64<0xB1>(v);         // quad_perm [1,0,3,2]
-      else if constexpr (D == 2) return dpp64<0x4E>(v);    // quad_perm [2,3,0,1]
-      else if constexpr (D == 4) return dpp64<0x141>(v);   // row_half_mirror
-      else return dpp64<0x140>(v);                         // row_mirror
+      if constexpr (D == 1) return dpp64<0xB1>(v, v);         // quad_perm [1,0,3,2]
+      else if constexpr (D == 2) return dpp64<0x4E>(v, v);    // quad_perm [2,3,0,1]
+      else if constexpr (D == 4) return dpp64<0x141>(v, v);   // row_half_mirror
+      else return dpp64<0x140>(v, v);                         // row_mirror
     } else {
       return __shfl_xor(v, D, LPM);
     }
   }
 };
 
-// coefficient rows of one mode in LDS, layout [c][row][g], row stride GS
-template <class T>
+// coefficient rows of one mode in LDS, layout [c][row][g] (rows padded to
+// LPM E with zeros, row stride GS); conj: read the conjugate
+template <class T, int NZP>
 struct LdsCoef {
   const typename C2<T>::t* S;
-  int GS, g, Nz, r0;
+  int GS, g, r0;
   bool conj;
   __device__ cplx operator()(int c, int q) const
   {
-    const int row = r0 + q;
-    if (row >= Nz) return mk(0, 0);
-    const typename C2<T>::t v = S[(c * Nz + row) * GS + g];
+    const typename C2<T>::t v = S[(c * NZP + r0 + q) * GS + g];
     return conj ? mk((double)v.x, -(double)v.y) : mk((double)v.x, (double)v.y);
   }
 };
 
-// store velocity coefficients; avg: average with the conjugate of v (the
-// mirror solve of a Nyquist-touching mode)
-template <class T>
+// store velocity coefficients (padding rows included); avg: average with the
+// conjugate of v (the mirror solve of a Nyquist-touching mode)
+template <class T, int NZP>
 struct LdsStore {
   typename C2<T>::t* S;
-  int GS, g, Nz, r0;
+  int GS, g, r0;
   bool avg;
   __device__ void operator()(int c, int q, cplx v)
   {
-    typename C2<T>::t& d = S[(c * Nz + r0 + q) * GS + g];
+    typename C2<T>::t& d = S[(c * NZP + r0 + q) * GS + g];
     if (avg) {
       d.x = (T)(0.5 * ((double)d.x + v.re));
       d.y = (T)(0.5 * ((double)d.y - v.im));
@@ -110,11 +141,11 @@ struct LdsStore {
   }
 };
 
-// Global-memory view of the coefficients of one mode (the stash that keeps the
-// forcing coefficients of a Nyquist-touching mode for its mirror solve).
+// Forcing coefficients of a special mode kept in its output column (element
+// (c, row) at p[(c Nz + row) Nm]), read conjugated for the mirror solve.
 template <class T>
-struct GlobalCoef {
-  const typename C2<T>::t* p;   // element (c, row) at p[(c * Nz + row) * Nm]
+struct StashCoef {
+  const typename C2<T>::t* p;
   long Nm;
   int Nz, r0;
   __device__ cplx operator()(int c, int q) const
@@ -122,69 +153,98 @@ struct GlobalCoef {
     const int row = r0 + q;
     if (row >= Nz) return mk(0, 0);
     const typename C2<T>::t v = p[(size_t)(c * Nz + row) * Nm];
-    return mk((double)v.x, -(double)v.y);   // conjugated input of the mirror
+    return mk((double)v.x, -(double)v.y);
   }
 };
 
-template <class T, int LPM, int E, int NRV>
-__global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<T>::t* __restrict__ in,
-                                                  typename C2<T>::t* __restrict__ out)
+__device__ __forceinline__ bool special_mode(int m, int Nx, int Ny, int Nkx)
+{
+  const int iy = m / Nkx, ix = m - iy * Nkx;
+  return m == 0 || ((Nx % 2 == 0) && ix == Nx / 2) || ((Ny % 2 == 0) && iy == Ny / 2);
+}
+
+// Modes per workgroup G and the waves per SIMD that LDS allows are fixed by
+// the shape, so LDS offsets are compile-time and the register budget of the
+// regular kernel is capped to reach that occupancy (the special kernel runs a
+// few dozen workgroups and gets no cap).
+template <class T, int LPM, int E, bool SPECIAL>
+__global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 4) * LPM,
+                                  SPECIAL ? 1 : spectral_waves_per_simd(LPM, E, sizeof(T) == 4))
+    k_spectral(SpecArgs A, const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out)
 {
   typedef typename C2<T>::t V;
-  constexpr int R = E / 2;   // node pairs per lane in the inverse transform
+  constexpr int R = E / 2;       // node pairs per lane in the inverse transform
+  constexpr int NZP = LPM * E;   // rows held per component (zero from Nz on)
+  constexpr int G = spectral_modes_per_block(LPM, E, sizeof(T) == 4);
+  constexpr int NT = G * LPM;    // threads per workgroup
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  const int Nz = A.B.Nz, G = A.G, Nm = A.Nm, NH = A.NH;
-  const int ncol = 3 * Nz * G;
+  const int Nz = A.B.Nz, Nm = A.Nm, NH = A.NH;
   // [c][row][g] with row stride G+1: the lanes of one mode (different rows)
   // fall in different LDS banks
-  const int GS = G + 1;
-  V* S = reinterpret_cast<V*>(smem_raw);   // values, coefficients, values
-  const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
+  constexpr int GS = G + 1;
+  double* tab = reinterpret_cast<double*>(smem_raw);   // [TB_N][NZP]
+  double2* Cr = reinterpret_cast<double2*>(tab + TB_N * NZP);   // [G][3][4] wall corrections
+  V* S = reinterpret_cast<V*>(Cr + 12 * G);            // values, coefficients, values
   const int g = threadIdx.x / LPM;
   const int l = threadIdx.x % LPM;
-  const int m = m0 + g;
   const int r0 = l * E;
-  int ix = 0, iy = 0, nyqx = 0, nyqy = 0;
-  if (m < Nm) {
-    iy = m / A.Nkx;
-    ix = m - iy * A.Nkx;
-    nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
-    nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+  // mode of this lane group (m) and column of this thread in phases 0 and 4
+  // (mc; blockDim.x is a multiple of G, so the column is fixed per thread);
+  // -1 = nothing to do
+  const int gc = threadIdx.x % G;
+  int m, mc;
+  if constexpr (SPECIAL) {
+    const int b0 = blockIdx.x * G;
+    m = (b0 + g < A.nspecial) ? A.special[b0 + g] : -1;
+    mc = (b0 + gc < A.nspecial) ? A.special[b0 + gc] : -1;
+  } else {
+    const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
+    m = (m0 + g < Nm && !special_mode(m0 + g, A.Nx, A.Ny, A.Nkx)) ? m0 + g : -1;
+    mc = (m0 + gc < Nm) ? m0 + gc : -1;
   }
-  const bool special = (m > 0) && (m < Nm) && (nyqx || nyqy);
 
-  // ---- phase 0: columns in ----
-  for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
-    const int gi = i % G, row = i / G;
+  // ---- phase 0: row tables and columns in ----
+  for (int i = threadIdx.x; i < TB_N * NZP; i += NT) {
+    const int t = i / NZP, row = i - t * NZP;
+    tab[i] = (row < Nz) ? row_table(A.B, t)[row] : 0.0;
+  }
+  for (int row = threadIdx.x / G; row < 3 * NZP; row += NT / G) {
+    const int c = row / NZP, r = row - c * NZP;
     V v;
     v.x = v.y = (T)0;
-    if (m0 + gi < Nm) v = in[(size_t)row * Nm + m0 + gi];
-    S[row * GS + gi] = v;
+    if (mc >= 0 && r < Nz) v = in[(size_t)(c * Nz + r) * Nm + mc];
+    S[row * GS + gc] = v;
   }
   __syncthreads();
 
   // ---- phase 1: forward transform of this lane's rows ----
   {
+    typedef T T2 __attribute__((ext_vector_type(2)));
     T ar[3][E], ai[3][E];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < E; ++q) ar[c][q] = ai[c][q] = (T)0;
-    const T* Tf = (const T*)A.TfP;
+    // [NH][LPM][E]: this lane's E rows of node pair j are contiguous
+    const T2* Tf = (const T2*)A.TfP + l * (E / 2);
 #pragma unroll 2
     for (int j = 0; j < NH; ++j) {
       T er[3], ei[3], orr[3], oi[3];
       const bool mid = (j == Nz - 1 - j);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const V a = S[(c * Nz + j) * GS + g];
-        const V b = S[(c * Nz + Nz - 1 - j) * GS + g];
+        const V a = S[(c * NZP + j) * GS + g];
+        const V b = S[(c * NZP + Nz - 1 - j) * GS + g];
         if (mid) { er[c] = a.x; ei[c] = a.y; orr[c] = oi[c] = (T)0; }
         else { er[c] = a.x + b.x; ei[c] = a.y + b.y; orr[c] = a.x - b.x; oi[c] = a.y - b.y; }
       }
       T t[E];
 #pragma unroll
-      for (int q = 0; q < E; ++q) t[q] = Tf[((size_t)j * E + q) * LPM + l];
+      for (int q = 0; q < E; q += 2) {
+        const T2 w = Tf[(size_t)j * (LPM * E / 2) + q / 2];
+        t[q] = w.x;
+        t[q + 1] = w.y;
+      }
 #pragma unroll
       for (int q = 0; q < E; ++q)
 #pragma unroll
@@ -196,115 +256,135 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q)
-      if (r0 + q < Nz)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          V v;
-          v.x = ar[c][q];
-          v.y = ai[c][q];
-          S[(c * Nz + r0 + q) * GS + g] = v;
-          if (special) out[(size_t)(c * Nz + r0 + q) * Nm + m] = v;
-        }
+      for (int c = 0; c < 3; ++c) {
+        V v;
+        v.x = ar[c][q];
+        v.y = ai[c][q];
+        S[(c * NZP + r0 + q) * GS + g] = v;
+        if constexpr (SPECIAL)
+          if (m >= 0 && r0 + q < Nz) out[(size_t)(c * Nz + r0 + q) * Nm + m] = v;
+      }
   }
+  __syncthreads();
 
   // ---- phase 2: FP64 BVP of mode m on this lane group, in place ----
-  ModeResult res;
-  res.nbasis = 0;
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) res.corr[c][b] = mk(0, 0);
+  // (the wall corrections go to LDS, not through registers to phase 3)
   double K = 0;
-  if (m < Nm) {
+  if (m >= 0) {
+    ModeResult res;
+    res.nbasis = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) res.corr[c][b] = mk(0, 0);
+    BvpConsts B = A.B;   // row tables from LDS
+    B.sd0 = tab + TB_SD0 * NZP;
+    B.sm2 = tab + TB_SM2 * NZP;
+    B.sp2 = tab + TB_SP2 * NZP;
+    B.fm1 = tab + TB_FM1 * NZP;
+    B.fp1 = tab + TB_FP1 * NZP;
+    B.R1 = tab + TB_R1 * NZP;
+    B.R2 = tab + TB_R2 * NZP;
+    B.L1 = tab + TB_L1 * NZP;
+    B.L2 = tab + TB_L2 * NZP;
     const DevLanes<LPM> ln{l};
-    LdsCoef<T> fin{S, GS, g, Nz, r0, false};
-    LdsStore<T> fout{S, GS, g, Nz, r0, false};
-    if (m == 0) {
-      seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);
+    LdsCoef<T, NZP> fin{S, GS, g, r0, false};
+    LdsStore<T, NZP> fout{S, GS, g, r0, false};
+    const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
+    if (SPECIAL && m == 0) {
+      seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);   // unpadded global tables
     } else {
-      const double kx = kvec(ix, A.Nx, A.Lx), ky = kvec(iy, A.Ny, A.Ly);
+      const double kx = A.kxs[ix], ky = A.kys[iy];
       K = sqrt(kx * kx + ky * ky);
+      // [Nm][LPM][E] factor rows of this lane, [Nm][4] inverse Schur complement
       double ib[E], ab[E], Gi[4];
+      const double2* pib = (const double2*)(A.fac_ib + ((size_t)m * LPM + l) * E);
+      const double2* pab = (const double2*)(A.fac_ab + ((size_t)m * LPM + l) * E);
 #pragma unroll
-      for (int q = 0; q < E; ++q) {
-        ib[q] = A.fac_ib[((size_t)m * E + q) * LPM + l];
-        ab[q] = A.fac_ab[((size_t)m * E + q) * LPM + l];
+      for (int q = 0; q < E; q += 2) {
+        const double2 u = pib[q / 2], w = pab[q / 2];
+        ib[q] = u.x; ib[q + 1] = u.y;
+        ab[q] = w.x; ab[q + 1] = w.y;
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Gi[i] = A.fac_G[(size_t)m * 4 + i];
-      seg_bvp_mode<LPM, E, NRV>(A.B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res);
-      if (special) {
-        // the reference's c2c transform keeps only the real part of the
-        // inverse: the Hermitian average of this mode and its mirror, solved
-        // with the mirror's own grid wavenumbers (same |k|, same factors) on
-        // the conjugated forcing kept in the output column
+      {
+        const double2* pg = (const double2*)(A.fac_G + (size_t)m * 4);
+        const double2 g0 = pg[0], g1 = pg[1];
+        Gi[0] = g0.x; Gi[1] = g0.y; Gi[2] = g1.x; Gi[3] = g1.y;
+      }
+      if constexpr (!SPECIAL) {
+        seg_bvp_mode<LPM, E, DPS_SPEC_NRV, true>(B, ln, kx, ky, 0, 0, ib, ab, Gi, fin, fout, res);
+      } else {
+        const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
+        const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+        seg_bvp_mode<LPM, E, DPS_SPEC_NRV>(B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res);
+        // mirror: the same |k| (same factors) with its own grid wavenumbers,
+        // on the conjugated forcing
         const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
         const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
         const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
         const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
-        // the first solve's correction waits in memory (every lane of the
-        // group writes the same values), not in registers
-        double2* cs = A.cstash + (size_t)m * 12;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) cs[c * 4 + b] = make_double2(res.corr[c][b].re, res.corr[c][b].im);
-        DPS_STAGE_FENCE();
-        GlobalCoef<T> fin2{out + m, (long)Nm, Nz, r0};
+        ModeResult res2;
+        StashCoef<T> fin2{out + m, (long)Nm, Nz, r0};
         fout.avg = true;
-        seg_bvp_mode<LPM, E, NRV>(A.B, ln, kxm, kym, nyqxm, nyqym, ib, ab, Gi, fin2, fout, res);
+        seg_bvp_mode<LPM, E, DPS_SPEC_NRV>(B, ln, kxm, kym, nyqxm, nyqym, ib, ab, Gi, fin2, fout, res2);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            const double2 a = cs[c * 4 + b];
-            res.corr[c][b] = mk(0.5 * (a.x + res.corr[c][b].re), 0.5 * (a.y - res.corr[c][b].im));
-          }
+          for (int b = 0; b < 4; ++b)
+            res.corr[c][b] = mk(0.5 * (res.corr[c][b].re + res2.corr[c][b].re),
+                                0.5 * (res.corr[c][b].im - res2.corr[c][b].im));
       }
     }
+    if (l == 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) Cr[g * 12 + c * 4 + b] = make_double2(res.corr[c][b].re, res.corr[c][b].im);
   }
   __syncthreads();
 
   // ---- phase 3: inverse transform + wall profiles ----
   {
-    const T* Ti = (const T*)A.TiP;
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    // [ceil(Nz/2)][LPM][E]: for the coefficient pair (2p, 2p+1), the R rows
+    // of coefficient 2p then the R rows of 2p+1, for this lane's node pairs
+    const T2* Ti = (const T2*)A.TiP + l * (E / 2);
     T er[3][R], ei[3][R], orr[3][R], oi[3][R];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < R; ++r) er[c][r] = ei[c][r] = orr[c][r] = oi[c][r] = (T)0;
 #pragma unroll 2
-    for (int k = 0; k < Nz; k += 2) {
-      V u[3];
+    for (int p = 0; p < Nz; p += 2) {
+      V u[3], w[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * GS + g];
+      for (int c = 0; c < 3; ++c) {
+        u[c] = S[(c * NZP + p) * GS + g];
+        w[c] = S[(c * NZP + p + 1) * GS + g];   // a zero padding row when p + 1 = Nz
+      }
+      T t[E];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const T t = Ti[((size_t)k * R + r) * LPM + l];
+      for (int q = 0; q < E; q += 2) {
+        const T2 v = Ti[(size_t)(p / 2) * (LPM * E / 2) + q / 2];
+        t[q] = v.x;
+        t[q + 1] = v.y;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          er[c][r] += t * u[c].x;
-          ei[c][r] += t * u[c].y;
+          er[c][r] += t[r] * u[c].x;
+          ei[c][r] += t[r] * u[c].y;
+          orr[c][r] += t[R + r] * w[c].x;
+          oi[c][r] += t[R + r] * w[c].y;
         }
-      }
-    }
-#pragma unroll 2
-    for (int k = 1; k < Nz; k += 2) {
-      V u[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = S[(c * Nz + k) * GS + g];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const T t = Ti[((size_t)k * R + r) * LPM + l];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          orr[c][r] += t * u[c].x;
-          oi[c][r] += t * u[c].y;
-        }
-      }
     }
     __syncthreads();
-    const int nb = res.nbasis;
+    // basis functions per component (bottom wall: 2, slit: 4; the k = 0
+    // mode has zero corrections)
+    const int nb = (A.B.geometry == MODE_BOTTOM) ? 2 : ((A.B.geometry == MODE_SLIT) ? 4 : 0);
+    const double2* cr = Cr + g * 12;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int j = l + LPM * r;
@@ -329,33 +409,38 @@ __global__ void __launch_bounds__(256) k_spectral(SpecArgs A, const typename C2<
           v.x = side ? (er[c][r] - orr[c][r]) : (er[c][r] + orr[c][r]);
           v.y = side ? (ei[c][r] - oi[c][r]) : (ei[c][r] + oi[c][r]);
           if (nb) {
-            double sr = res.corr[c][0].re * e1, si = res.corr[c][0].im * e1;
+            const double2 c0 = cr[c * 4], c1 = cr[c * 4 + 1];
+            double sr = c0.x * e1, si = c0.y * e1;
             if (nb == 2) {
-              sr += res.corr[c][1].re * (z * e1);
-              si += res.corr[c][1].im * (z * e1);
+              sr += c1.x * (z * e1);
+              si += c1.y * (z * e1);
             } else {
-              sr += res.corr[c][1].re * e2 + res.corr[c][2].re * (z * e1) + res.corr[c][3].re * (z * e2);
-              si += res.corr[c][1].im * e2 + res.corr[c][2].im * (z * e1) + res.corr[c][3].im * (z * e2);
+              const double2 c2 = cr[c * 4 + 2], c3 = cr[c * 4 + 3];
+              sr += c1.x * e2 + c2.x * (z * e1) + c3.x * (z * e2);
+              si += c1.y * e2 + c2.y * (z * e1) + c3.y * (z * e2);
             }
             v.x += (T)(sr * A.inv_nxy);
             v.y += (T)(si * A.inv_nxy);
           }
-          S[(c * Nz + jj) * GS + g] = v;
+          S[(c * NZP + jj) * GS + g] = v;
         }
       }
     }
   }
   __syncthreads();
 
-  // ---- phase 4: columns out ----
-  for (int i = threadIdx.x; i < ncol; i += blockDim.x) {
-    const int gi = i % G, row = i / G;
-    if (m0 + gi < Nm) out[(size_t)row * Nm + m0 + gi] = S[row * GS + gi];
-  }
+  // ---- phase 4: columns out (the regular kernel leaves special columns) ----
+  const bool write = (mc >= 0) && (SPECIAL || !special_mode(mc, A.Nx, A.Ny, A.Nkx));
+  if (write)
+    for (int row = threadIdx.x / G; row < 3 * Nz; row += NT / G) {
+      const int c = row / Nz, r = row - c * Nz;
+      out[(size_t)row * Nm + mc] = S[(c * NZP + r) * GS + gc];
+    }
 }
 
 // Static factorisation of every mode (depends on |k| only), one lane per
-// mode, written in the lane order of k_spectral.
+// mode, written in the lane order of k_spectral: fib/fab [Nm][lpm][e],
+// fG [Nm][4].
 __global__ void k_bvp_factor(BvpConsts B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
                              double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab,
                              double* fG)
@@ -375,8 +460,7 @@ __global__ void k_bvp_factor(BvpConsts B, int Nx, int Ny, int Nkx, int Nm, doubl
   }
   const int Nz = B.Nz;
   for (int row = 0; row < lpm * e; ++row) {
-    const int l = row / e, q = row - l * e;
-    const size_t i = ((size_t)m * e + q) * lpm + l;
+    const size_t i = (size_t)m * lpm * e + row;   // row = l e + q
     const bool live = (m != 0) && (row < Nz);
     fib[i] = live ? s_ib[(size_t)(row + 1) * Nm + m] : 0.0;
     fab[i] = live ? s_ab[(size_t)row * Nm + m] : 0.0;
@@ -395,13 +479,7 @@ bool spectral_shape(int Nz, int* lpm, int* e)
   return false;
 }
 
-int spectral_modes_per_block(int Nz, int lpm, bool mixed)
-{
-  const size_t per_row = (size_t)3 * Nz * (mixed ? sizeof(float2) : sizeof(double2));
-  int G = 256 / lpm;
-  while (G > 1 && (size_t)(G + 1) * per_row > 65536) G /= 2;
-  return G;
-}
+static_assert(TB_N == 9, "spectral_lds_bytes counts 9 row tables");
 
 void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
                        double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,
@@ -412,20 +490,28 @@ void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, doub
 }
 
 template <class T>
-void launch_spectral(const SpecArgs& A, const void* in, void* out, hipStream_t s)
+void launch_spectral(const SpecArgs& A0, const void* in, void* out, hipStream_t s)
 {
   typedef typename C2<T>::t V;
-  const size_t lds = (size_t)3 * A.B.Nz * (A.G + 1) * sizeof(V);
-  const dim3 grid((A.Nm + A.G - 1) / A.G), block(A.G * A.lpm);
+  const bool mixed = sizeof(T) == sizeof(float);
   const V* i = (const V*)in;
   V* o = (V*)out;
-#define SPEC_CASE(LP, EE)                                                                    \
-  if (A.lpm == LP && A.e == EE) {                                                            \
-    hipLaunchKernelGGL((k_spectral<T, LP, EE, DPS_SPEC_NRV>), grid, block, lds, s, A, i, o); \
-    return;                                                                                  \
+  const SpecArgs& A = A0;
+  const int G = spectral_modes_per_block(A.lpm, A.e, mixed);
+  const size_t lds = spectral_lds_bytes(A.lpm, A.e, G, mixed);
+  for (int special = 0; special < 2; ++special) {
+    if (special && A.nspecial == 0) break;
+    const int work = special ? A.nspecial : A.Nm;
+    const dim3 grid((work + G - 1) / G), block(G * A.lpm);
+#define SPEC_CASE(LP, EE)                                                                         \
+  if (A.lpm == LP && A.e == EE) {                                                                 \
+    if (special) hipLaunchKernelGGL((k_spectral<T, LP, EE, true>), grid, block, lds, s, A, i, o); \
+    else hipLaunchKernelGGL((k_spectral<T, LP, EE, false>), grid, block, lds, s, A, i, o);        \
+    continue;                                                                                     \
   }
-  SPEC_CASE(16, 2) SPEC_CASE(16, 4) SPEC_CASE(16, 6) SPEC_CASE(16, 8) SPEC_CASE(32, 8) SPEC_CASE(64, 8)
+    SPEC_CASE(16, 2) SPEC_CASE(16, 4) SPEC_CASE(16, 6) SPEC_CASE(16, 8) SPEC_CASE(32, 8) SPEC_CASE(64, 8)
 #undef SPEC_CASE
+  }
 }
 
 template void launch_spectral<float>(const SpecArgs&, const void*, void*, hipStream_t);

@@ -5,6 +5,7 @@
 #include <barrier>
 #include <cstring>
 #include <thread>
+#include <algorithm>
 #include <vector>
 #include "dps_core.h"
 #include "dps_seg.h"
@@ -52,8 +53,16 @@ struct HostLanes {
     g->bar.arrive_and_wait();
     return r;
   }
-  template <int D> double up(double v) const { return exch(v, l - D >= 0 ? l - D : l); }
-  template <int D> double down(double v) const { return exch(v, l + D < n ? l + D : l); }
+  template <int D> double up(double v, double fill) const
+  {
+    const double r = exch(v, l - D >= 0 ? l - D : l);
+    return l - D >= 0 ? r : fill;
+  }
+  template <int D> double down(double v, double fill) const
+  {
+    const double r = exch(v, l + D < n ? l + D : l);
+    return l + D < n ? r : fill;
+  }
   template <int D> double xr(double v) const { return exch(v, l ^ D); }
 };
 struct SegIn {
@@ -72,17 +81,40 @@ struct SegOut {
   int Nz, r0;
   void operator()(int c, int q, cplx v)
   {
+    if (r0 + q >= Nz) return;   // padding rows
     double* o = p + 2 * (c * Nz + r0 + q);
     o[0] = v.re;
     o[1] = v.im;
   }
 };
 
-template <int LPM, int E, int NRV>
+// Copies of the row tables of B readable at every row a lane group touches,
+// zero from row Nz on (what seg_bvp_mode requires).
+struct PaddedTables {
+  std::vector<double> t[9];
+  BvpConsts B;
+  PaddedTables(const BvpConsts& B0, int rows)
+  {
+    B = B0;
+    const double* src[9] = {B0.sd0, B0.sm2, B0.sp2, B0.fm1, B0.fp1, B0.R1, B0.R2, B0.L1, B0.L2};
+    const int n = std::max(rows, B0.Nz + 2);
+    for (int k = 0; k < 9; ++k) {
+      t[k].assign(n, 0.0);
+      for (int r = 0; r < B0.Nz; ++r) t[k][r] = src[k][r];
+    }
+    B.sd0 = t[0].data(); B.sm2 = t[1].data(); B.sp2 = t[2].data(); B.fm1 = t[3].data();
+    B.fp1 = t[4].data(); B.R1 = t[5].data(); B.R2 = t[6].data(); B.L1 = t[7].data(); B.L2 = t[8].data();
+  }
+};
+
+// STAGE: fin and fout address one buffer (the forcing is copied to uout
+// first), as in the kernel's in-place column.
+template <int LPM, int E, int NRV, bool STAGE>
 void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, const int* nyqx,
              const int* nyqy, const double* fin, double* uout, double* corr)
 {
   const int Nz = B.Nz;
+  const PaddedTables pad(B, LPM * E);
   std::vector<double> ibv(Nz + 1), abv(Nz);
   std::vector<cplx> dummy(Nz);
   ScratchView sc;
@@ -94,6 +126,7 @@ void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, cons
     if (!k0) bvp_factor(B, kx[m] * kx[m] + ky[m] * ky[m], sc, Gi);
     HostGroup grp(LPM);
     ModeResult res0;
+    if (STAGE) std::copy(fin + (size_t)m * 6 * Nz, fin + (size_t)(m + 1) * 6 * Nz, uout + (size_t)m * 6 * Nz);
     std::vector<std::thread> th;
     for (int l = 0; l < LPM; ++l)
       th.emplace_back([&, l]() {
@@ -104,14 +137,14 @@ void run_seg(const BvpConsts& B, int M, const double* kx, const double* ky, cons
           ib[q] = (row < Nz && !k0) ? ibv[row + 1] : 0.0;
           ab[q] = (row < Nz && !k0) ? abv[row] : 0.0;
         }
-        SegIn in{fin + (size_t)m * 6 * Nz, Nz, l * E};
+        SegIn in{(STAGE ? uout : fin) + (size_t)m * 6 * Nz, Nz, l * E};
         SegOut out{uout + (size_t)m * 6 * Nz, Nz, l * E};
         ModeResult r;
         for (int c = 0; c < 3; ++c)
           for (int b = 0; b < 4; ++b) r.corr[c][b] = mk(0, 0);
         r.nbasis = 0;
         if (k0) seg_bvp_k0<LPM, E>(B, ln, in, out);
-        else seg_bvp_mode<LPM, E, NRV>(B, ln, kx[m], ky[m], nyqx[m], nyqy[m], ib, ab, Gi, in, out, r);
+        else seg_bvp_mode<LPM, E, NRV, STAGE>(pad.B, ln, kx[m], ky[m], nyqx[m], nyqy[m], ib, ab, Gi, in, out, r);
         if (l == 0) res0 = r;
       });
     for (auto& t : th) t.join();
@@ -239,17 +272,18 @@ void cc_bvp(int Nz, double H, double Hs, double eta, int geometry, int M, const 
 // unsupported shape.
 int cc_bvp_seg(int Nz, double H, double Hs, double eta, int geometry, int M, const double* kx,
                const double* ky, const int* nyqx, const int* nyqy, const double* fin, double* uout,
-               double* corr, int lpm, int e, int nrv)
+               double* corr, int lpm, int e, int nrv, int stage)
 {
   ChebTables T = build_cheb_tables(Nz, H);
   BvpConsts B = host_consts(T, Nz, H, Hs, eta, geometry);
   if (lpm * e < Nz) return -1;
-#define SEG_CASE(LP, EE)                                                                     \
-  if (lpm == LP && e == EE && nrv == 3) { run_seg<LP, EE, 3>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; } \
-  if (lpm == LP && e == EE && nrv == 1) { run_seg<LP, EE, 1>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; }
+#define SEG_RUN(LP, EE, NR, ST) \
+  if (lpm == LP && e == EE && nrv == NR && stage == ST) { run_seg<LP, EE, NR, ST>(B, M, kx, ky, nyqx, nyqy, fin, uout, corr); return 0; }
+#define SEG_CASE(LP, EE) SEG_RUN(LP, EE, 3, 0) SEG_RUN(LP, EE, 1, 0) SEG_RUN(LP, EE, 1, 1) SEG_RUN(LP, EE, 3, 1)
   SEG_CASE(4, 2) SEG_CASE(4, 4) SEG_CASE(8, 4) SEG_CASE(16, 2) SEG_CASE(16, 4) SEG_CASE(16, 6)
   SEG_CASE(16, 8) SEG_CASE(32, 8)
 #undef SEG_CASE
+#undef SEG_RUN
   return -1;
 }
 }

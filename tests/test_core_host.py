@@ -81,16 +81,17 @@ def test_sequential_bvp_matches_oracle(corecheck, geometry):
     assert np.abs(tot - U).max() <= 1e-12 * np.abs(U).max()
 
 
-SHAPES = [(17, 16, 2, 3), (33, 16, 4, 1), (61, 16, 4, 3), (96, 16, 6, 1), (128, 16, 8, 1),
-          (150, 32, 8, 1), (13, 4, 4, 3)]
+SHAPES = [(17, 16, 2, 3, 0), (33, 16, 4, 1, 1), (61, 16, 4, 3, 0), (96, 16, 6, 1, 1), (96, 16, 6, 1, 0),
+          (128, 16, 8, 1, 1), (150, 32, 8, 1, 1), (13, 4, 4, 3, 1)]
 
 
 @pytest.mark.parametrize("geometry", ["nowall", "bottom", "slit"])
-@pytest.mark.parametrize("shape", SHAPES, ids=["nz%d_lpm%d_e%d_nrv%d" % s for s in SHAPES])
+@pytest.mark.parametrize("shape", SHAPES, ids=["nz%d_lpm%d_e%d_nrv%d_stage%d" % s for s in SHAPES])
 def test_lane_group_bvp_matches_sequential(corecheck, geometry, shape):
     """seg_bvp_mode/seg_bvp_k0 (LPM lanes of E rows, affine scans across
-    lanes) against the sequential solver, padding rows included."""
-    Nz, lpm, e, nrv = shape
+    lanes) against the sequential solver, padding rows included; stage = the
+    in-place variant that stages the x/y right-hand sides in the forcing."""
+    Nz, lpm, e, nrv, stage = shape
     Nx, Ny, Lx, Ly, Lz, eta = 4, 3, 8.0, 6.0, 5.0, 0.8
     kx, ky, nyqx, nyqy, Cf, fin = mode_inputs(Nx, Ny, Nz, Lx, Ly, Nz)
     M = Nx * Ny
@@ -98,7 +99,7 @@ def test_lane_group_bvp_matches_sequential(corecheck, geometry, shape):
     c1, c2 = np.zeros((M, 3, 4, 2)), np.zeros((M, 3, 4, 2))
     args = (Nz, D(Lz / 2), D(Lz), D(eta), GEOM[geometry], M, P(kx), P(ky), P(nyqx), P(nyqy), P(fin))
     corecheck.cc_bvp(*args, P(u1), P(c1))
-    assert corecheck.cc_bvp_seg(*args, P(u2), P(c2), lpm, e, nrv) == 0
+    assert corecheck.cc_bvp_seg(*args, P(u2), P(c2), lpm, e, nrv, stage) == 0
     assert np.abs(u2 - u1).max() <= 1e-13 * np.abs(u1).max()
     assert np.abs(c2 - c1).max() <= 1e-13 * max(np.abs(c1).max(), 1e-300)
 
