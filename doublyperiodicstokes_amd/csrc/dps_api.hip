@@ -400,16 +400,7 @@ int allocate_fused(dps_handle* h)
         ti[((size_t)(k / 2) * LPM + l) * E + (k & 1) * R + r] =
             inv_nxy * ((k & 1) ? h->ct.TiO[(size_t)j * h->ct.NO + k / 2] : h->ct.TiE[(size_t)j * h->ct.NE + k / 2]);
       }
-  // k = 0 and the modes on a Nyquist index, solved by the special kernel
-  std::vector<int> sp;
-  for (int iy = 0; iy < h->Ny; ++iy)
-    for (int ix = 0; ix < h->Nkx; ++ix) {
-      const bool s = (ix == 0 && iy == 0) || ((h->Nx % 2 == 0) && ix == h->Nx / 2) ||
-                     ((h->Ny % 2 == 0) && iy == h->Ny / 2);
-      if (s) sp.push_back(iy * h->Nkx + ix);
-    }
-  h->nspecial = (int)sp.size();
-  if ((rc = h->upload(&h->d_special, sp))) return rc;
+
   if (h->prec == DPS_PREC_MIXED) {
     if ((rc = h->upload((float**)&h->d_TfP, cast_vec<float>(tf)))) return rc;
     if ((rc = h->upload((float**)&h->d_TiP, cast_vec<float>(ti)))) return rc;
@@ -589,7 +580,6 @@ SpecArgs spec_args(dps_handle* h)
   A.TfP = h->d_TfP; A.TiP = h->d_TiP;
   A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
   A.kxs = h->d_kxs; A.kys = h->d_kys;
-  A.special = h->d_special; A.nspecial = h->nspecial;
   A.zpts = h->d_zpts;
   A.Hs = h->Lz;
   A.inv_nxy = 1.0 / ((double)h->Nx * (double)h->Ny);
@@ -622,14 +612,13 @@ int do_solve(dps_handle* h, hipStream_t s, int ev0)
   if ((rc = run_fft(h, true, h->d_grid, h->d_specA, s))) return rc;
   mark(h, ev0 + 1, s);
   if (h->fused) {
-    // z transforms are inside the fused launch: their stage times read zero
-    mark(h, ev0 + 2, s);
+    // z transforms are inside the fused launch: no events of their own
+    // (dps_get_timings reports them as zero)
     const SpecArgs A = spec_args(h);
     if (mixed(h)) launch_spectral<float>(A, h->d_specA, h->d_specB, s);
     else launch_spectral<double>(A, h->d_specA, h->d_specB, s);
     HIPCHK(hipGetLastError());
     mark(h, ev0 + 3, s);
-    mark(h, ev0 + 4, s);
     if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
     mark(h, ev0 + 5, s);
     return DPS_OK;
@@ -858,10 +847,20 @@ int dps_get_timings(dps_handle* h, double* out)
   if (!h->have_timings) return fail(DPS_ERR_STATE, "no profiled Mdot available");
   HIPCHK(hipEventSynchronize(h->ev[7]));
   // events: 0 start, 1 spread done, 2 fft fwd, 3 z fwd, 4 bvp, 5 z inv, 6 fft inv, 7 interp
+  // (fused spectral kernel: 3 and 5 are not recorded; bvp spans 2 -> 4,
+  // fft inverse 4 -> 6)
   out[0] = 0.0;   // locate/sort happens in dps_set_positions
   for (int i = 0; i < 7; ++i) {
+    int a = i, b = i + 1;
+    if (h->fused) {
+      if (i == 2 || i == 4) {
+        out[i + 1] = 0.0;
+        continue;
+      }
+      if (i == 3 || i == 5) a = i - 1;
+    }
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[a], h->ev[b]));
     out[i + 1] = 1000.0 * ms;
   }
   return DPS_OK;

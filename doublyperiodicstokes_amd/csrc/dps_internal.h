@@ -124,8 +124,6 @@ struct SpecArgs {
   const double* fac_ab;   // [Nm][LPM][E] A^{-1}B of each lane's rows
   const double* fac_G;    // [Nm][4] inverse Schur complement
   const double *kxs, *kys;   // grid wavenumbers by index
-  const int* special;     // k = 0 and the Nyquist-touching modes
-  int nspecial;
   const double* zpts;
   double Hs, inv_nxy;
 };

@@ -259,6 +259,10 @@ DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y, cplx Y0, cplx Y
 // at row l*E+q (zero for rows >= Nz); fout(c, q, v) receives the velocity
 // coefficients; res gets the wall correction (identical on every lane of the
 // group).
+// avg (modes on a Nyquist index): the corrections are the mean of those at
+// (kx, ky) and at (kxa, kya) = minus the mirror mode's grid wavenumbers, from
+// the same wall values -- the Hermitian average of the mode and its mirror,
+// whose BVP part is the conjugate of the mode's own.
 // STAGE: the x and y velocity right-hand sides (D_c p - f_c) / eta are formed
 // as soon as the pressure is known and written through fout over the x and y
 // forcing, then read back through fin by their solves, so the pressure
@@ -267,7 +271,7 @@ DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y, cplx Y0, cplx Y
 template <int LPM, int E, int NRV = 3, bool STAGE = false, class L, class FIn, class FOut>
 DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, int nyq_x, int nyq_y,
                          const double (&ib)[E], const double (&ab)[E], const double* Gi, const FIn& fin,
-                         FOut& fout, ModeResult& res)
+                         FOut& fout, ModeResult& res, bool avg = false, double kxa = 0, double kya = 0)
 {
   const int Nz = B.Nz;
   const double H = B.H;
@@ -479,6 +483,15 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
 #undef C0
 #undef C1
   wall_correction(B, kx, ky, wall_bot, wall_top, res);
+  if (avg) {
+    ModeResult r2;
+    wall_correction(B, kxa, kya, wall_bot, wall_top, r2);
+    #pragma unroll
+    for (int c = 0; c < 3; ++c)
+      #pragma unroll
+      for (int b = 0; b < 4; ++b)
+        res.corr[c][b] = mk(0.5 * (res.corr[c][b].re + r2.corr[c][b].re), 0.5 * (res.corr[c][b].im + r2.corr[c][b].im));
+  }
 }
 
 // k = 0 mode (counterpart of bvp_k0).

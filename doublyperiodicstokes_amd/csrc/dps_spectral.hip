@@ -20,14 +20,11 @@
 // The spectrum crosses HBM once in each direction; everything in between
 // stays in LDS and registers.
 //
-// Two instantiations share this code.  The regular kernel takes G consecutive
-// modes per workgroup and solves every mode except k = 0 and the modes on a
-// Nyquist index, whose columns it leaves untouched.  The special kernel takes
-// those from a list: k = 0 gets its own solve, and a Nyquist-touching mode is
-// solved twice -- as itself and as its mirror on the conjugated forcing, kept
-// meanwhile in the mode's own output column -- and averaged, because the
-// reference's c2c transform keeps only the real part of the inverse.  The
-// regular kernel thus carries no code or registers for the rare cases.
+// Modes on a Nyquist index take the Hermitian average with their mirror mode
+// (the reference's c2c transform keeps only the real part of the inverse).
+// The mirror's BVP solution is the conjugate of the mode's own, so the
+// average costs one solve and a second evaluation of the wall corrections
+// (dps_seg.h, seg_bvp_mode's avg).
 #include <hip/hip_runtime.h>
 #include "dps_internal.h"
 #include "dps_seg.h"
@@ -108,68 +105,37 @@ struct DevLanes {
 };
 
 // coefficient rows of one mode in LDS, layout [c][row][g] (rows padded to
-// LPM E with zeros, row stride GS); conj: read the conjugate
-template <class T, int NZP>
+// LPM E with zeros, row stride GS)
+template <class T, int NZP, int GS>
 struct LdsCoef {
   const typename C2<T>::t* S;
-  int GS, g, r0;
-  bool conj;
+  int g, r0;
   __device__ cplx operator()(int c, int q) const
   {
     const typename C2<T>::t v = S[(c * NZP + r0 + q) * GS + g];
-    return conj ? mk((double)v.x, -(double)v.y) : mk((double)v.x, (double)v.y);
+    return mk((double)v.x, (double)v.y);
   }
 };
 
-// store velocity coefficients (padding rows included); avg: average with the
-// conjugate of v (the mirror solve of a Nyquist-touching mode)
-template <class T, int NZP>
+// store velocity coefficients (padding rows included)
+template <class T, int NZP, int GS>
 struct LdsStore {
   typename C2<T>::t* S;
-  int GS, g, r0;
-  bool avg;
+  int g, r0;
   __device__ void operator()(int c, int q, cplx v)
   {
     typename C2<T>::t& d = S[(c * NZP + r0 + q) * GS + g];
-    if (avg) {
-      d.x = (T)(0.5 * ((double)d.x + v.re));
-      d.y = (T)(0.5 * ((double)d.y - v.im));
-    } else {
-      d.x = (T)v.re;
-      d.y = (T)v.im;
-    }
+    d.x = (T)v.re;
+    d.y = (T)v.im;
   }
 };
 
-// Forcing coefficients of a special mode kept in its output column (element
-// (c, row) at p[(c Nz + row) Nm]), read conjugated for the mirror solve.
-template <class T>
-struct StashCoef {
-  const typename C2<T>::t* p;
-  long Nm;
-  int Nz, r0;
-  __device__ cplx operator()(int c, int q) const
-  {
-    const int row = r0 + q;
-    if (row >= Nz) return mk(0, 0);
-    const typename C2<T>::t v = p[(size_t)(c * Nz + row) * Nm];
-    return mk((double)v.x, -(double)v.y);
-  }
-};
-
-__device__ __forceinline__ bool special_mode(int m, int Nx, int Ny, int Nkx)
-{
-  const int iy = m / Nkx, ix = m - iy * Nkx;
-  return m == 0 || ((Nx % 2 == 0) && ix == Nx / 2) || ((Ny % 2 == 0) && iy == Ny / 2);
-}
-
-// Modes per workgroup G and the waves per SIMD that LDS allows are fixed by
-// the shape, so LDS offsets are compile-time and the register budget of the
-// regular kernel is capped to reach that occupancy (the special kernel runs a
-// few dozen workgroups and gets no cap).
-template <class T, int LPM, int E, bool SPECIAL>
+// The solve: each workgroup takes G consecutive modes (G and the waves per
+// SIMD that LDS allows are fixed by the shape, so LDS offsets are
+// compile-time and the register budget is capped to reach that occupancy).
+template <class T, int LPM, int E>
 __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 4) * LPM,
-                                  SPECIAL ? 1 : spectral_waves_per_simd(LPM, E, sizeof(T) == 4))
+                                  spectral_waves_per_simd(LPM, E, sizeof(T) == 4))
     k_spectral(SpecArgs A, const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out)
 {
   typedef typename C2<T>::t V;
@@ -190,18 +156,11 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   const int r0 = l * E;
   // mode of this lane group (m) and column of this thread in phases 0 and 4
   // (mc; blockDim.x is a multiple of G, so the column is fixed per thread);
-  // -1 = nothing to do
+  // -1 = beyond the last mode
   const int gc = threadIdx.x % G;
-  int m, mc;
-  if constexpr (SPECIAL) {
-    const int b0 = blockIdx.x * G;
-    m = (b0 + g < A.nspecial) ? A.special[b0 + g] : -1;
-    mc = (b0 + gc < A.nspecial) ? A.special[b0 + gc] : -1;
-  } else {
-    const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
-    m = (m0 + g < Nm && !special_mode(m0 + g, A.Nx, A.Ny, A.Nkx)) ? m0 + g : -1;
-    mc = (m0 + gc < Nm) ? m0 + gc : -1;
-  }
+  const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
+  const int m = (m0 + g < Nm) ? m0 + g : -1;
+  const int mc = (m0 + gc < Nm) ? m0 + gc : -1;
 
   // ---- phase 0: row tables and columns in ----
   for (int i = threadIdx.x; i < TB_N * NZP; i += NT) {
@@ -262,8 +221,6 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
         v.x = ar[c][q];
         v.y = ai[c][q];
         S[(c * NZP + r0 + q) * GS + g] = v;
-        if constexpr (SPECIAL)
-          if (m >= 0 && r0 + q < Nz) out[(size_t)(c * Nz + r0 + q) * Nm + m] = v;
       }
   }
   __syncthreads();
@@ -289,10 +246,10 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
     B.L1 = tab + TB_L1 * NZP;
     B.L2 = tab + TB_L2 * NZP;
     const DevLanes<LPM> ln{l};
-    LdsCoef<T, NZP> fin{S, GS, g, r0, false};
-    LdsStore<T, NZP> fout{S, GS, g, r0, false};
+    LdsCoef<T, NZP, GS> fin{S, g, r0};
+    LdsStore<T, NZP, GS> fout{S, g, r0};
     const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
-    if (SPECIAL && m == 0) {
+    if (m == 0) {
       seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);   // unpadded global tables
     } else {
       const double kx = A.kxs[ix], ky = A.kys[iy];
@@ -312,29 +269,13 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
         const double2 g0 = pg[0], g1 = pg[1];
         Gi[0] = g0.x; Gi[1] = g0.y; Gi[2] = g1.x; Gi[3] = g1.y;
       }
-      if constexpr (!SPECIAL) {
-        seg_bvp_mode<LPM, E, DPS_SPEC_NRV, true>(B, ln, kx, ky, 0, 0, ib, ab, Gi, fin, fout, res);
-      } else {
-        const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
-        const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
-        seg_bvp_mode<LPM, E, DPS_SPEC_NRV>(B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res);
-        // mirror: the same |k| (same factors) with its own grid wavenumbers,
-        // on the conjugated forcing
-        const int ixm = (A.Nx - ix) % A.Nx, iym = (A.Ny - iy) % A.Ny;
-        const double kxm = kvec(ixm, A.Nx, A.Lx), kym = kvec(iym, A.Ny, A.Ly);
-        const int nyqxm = (A.Nx % 2 == 0) && (ixm == A.Nx / 2);
-        const int nyqym = (A.Ny % 2 == 0) && (iym == A.Ny / 2);
-        ModeResult res2;
-        StashCoef<T> fin2{out + m, (long)Nm, Nz, r0};
-        fout.avg = true;
-        seg_bvp_mode<LPM, E, DPS_SPEC_NRV>(B, ln, kxm, kym, nyqxm, nyqym, ib, ab, Gi, fin2, fout, res2);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            res.corr[c][b] = mk(0.5 * (res.corr[c][b].re + res2.corr[c][b].re),
-                                0.5 * (res.corr[c][b].im - res2.corr[c][b].im));
-      }
+      const int nyqx = (A.Nx % 2 == 0) && (ix == A.Nx / 2);
+      const int nyqy = (A.Ny % 2 == 0) && (iy == A.Ny / 2);
+      // the mirror mode's grid wavenumbers, negated: the mirror index of a
+      // Nyquist index is itself, any other index maps to minus its wavenumber
+      const double kxa = nyqx ? 0.0 - kx : kx, kya = nyqy ? 0.0 - ky : ky;
+      seg_bvp_mode<LPM, E, DPS_SPEC_NRV, true>(B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res,
+                                                nyqx || nyqy, kxa, kya);
     }
     if (l == 0)
 #pragma unroll
@@ -429,9 +370,8 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   }
   __syncthreads();
 
-  // ---- phase 4: columns out (the regular kernel leaves special columns) ----
-  const bool write = (mc >= 0) && (SPECIAL || !special_mode(mc, A.Nx, A.Ny, A.Nkx));
-  if (write)
+  // ---- phase 4: columns out ----
+  if (mc >= 0)
     for (int row = threadIdx.x / G; row < 3 * Nz; row += NT / G) {
       const int c = row / Nz, r = row - c * Nz;
       out[(size_t)row * Nm + mc] = S[(c * NZP + r) * GS + gc];
@@ -499,19 +439,14 @@ void launch_spectral(const SpecArgs& A0, const void* in, void* out, hipStream_t 
   const SpecArgs& A = A0;
   const int G = spectral_modes_per_block(A.lpm, A.e, mixed);
   const size_t lds = spectral_lds_bytes(A.lpm, A.e, G, mixed);
-  for (int special = 0; special < 2; ++special) {
-    if (special && A.nspecial == 0) break;
-    const int work = special ? A.nspecial : A.Nm;
-    const dim3 grid((work + G - 1) / G), block(G * A.lpm);
-#define SPEC_CASE(LP, EE)                                                                         \
-  if (A.lpm == LP && A.e == EE) {                                                                 \
-    if (special) hipLaunchKernelGGL((k_spectral<T, LP, EE, true>), grid, block, lds, s, A, i, o); \
-    else hipLaunchKernelGGL((k_spectral<T, LP, EE, false>), grid, block, lds, s, A, i, o);        \
-    continue;                                                                                     \
+  const dim3 grid((A.Nm + G - 1) / G), block(G * A.lpm);
+#define SPEC_CASE(LP, EE)                                                         \
+  if (A.lpm == LP && A.e == EE) {                                                 \
+    hipLaunchKernelGGL((k_spectral<T, LP, EE>), grid, block, lds, s, A, i, o); \
+    return;                                                                       \
   }
-    SPEC_CASE(16, 2) SPEC_CASE(16, 4) SPEC_CASE(16, 6) SPEC_CASE(16, 8) SPEC_CASE(32, 8) SPEC_CASE(64, 8)
+  SPEC_CASE(16, 2) SPEC_CASE(16, 4) SPEC_CASE(16, 6) SPEC_CASE(16, 8) SPEC_CASE(32, 8) SPEC_CASE(64, 8)
 #undef SPEC_CASE
-  }
 }
 
 template void launch_spectral<float>(const SpecArgs&, const void*, void*, hipStream_t);

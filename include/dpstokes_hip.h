@@ -117,7 +117,8 @@ int dps_get_info(const dps_handle* h, int* out);
 /* Per-kernel timings of the last dps_mdot in microseconds (HIP events on the
    call's stream, enabled by dps_set_profiling).  `out` holds 8 doubles:
    locate+sort, spread, fft forward, z forward, bvp, z inverse, fft inverse,
-   interp. Returns DPS_ERR_STATE if profiling is off. */
+   interp; z forward and z inverse read 0 where one fused kernel runs them
+   with the bvp.  Returns DPS_ERR_STATE if profiling is off. */
 int dps_set_profiling(dps_handle* h, int enable);
 int dps_get_timings(dps_handle* h, double* out);
 

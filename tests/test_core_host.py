@@ -81,6 +81,49 @@ def test_sequential_bvp_matches_oracle(corecheck, geometry):
     assert np.abs(tot - U).max() <= 1e-12 * np.abs(U).max()
 
 
+@pytest.mark.parametrize("geometry", ["nowall", "bottom", "slit"])
+def test_nyquist_mirror_is_a_sign_flip_of_the_wall_correction(corecheck, geometry):
+    """The Hermitian average of a Nyquist-touching mode with its mirror
+    (solved on the conjugated forcing with the mirror's grid wavenumbers)
+    needs only one BVP solve: the mirror's velocity is the conjugate of the
+    mode's own, and its conjugated wall correction is the correction of the
+    mode's forcing at wavenumbers (-kx_mirror, -ky_mirror), whose BVP part is
+    again the mode's own.  Checked with the sequential solver."""
+    Nx, Ny, Nz, Lx, Ly, Lz, eta = 8, 6, 17, 8.0, 6.0, 5.0, 0.8
+    kx, ky, nyqx, nyqy, Cf, fin = mode_inputs(Nx, Ny, Nz, Lx, Ly, 11)
+    ix = np.tile(np.arange(Nx), Ny)
+    iy = np.repeat(np.arange(Ny), Nx)
+    sel = np.where(((nyqx == 1) | (nyqy == 1)) & ~((ix == 0) & (iy == 0)))[0]
+    ixm, iym = (Nx - ix[sel]) % Nx, (Ny - iy[sel]) % Ny
+    kxm = 2 * np.pi * np.where(ixm <= Nx // 2, ixm, ixm - Nx) / Lx
+    kym = 2 * np.pi * np.where(iym <= Ny // 2, iym, iym - Ny) / Ly
+    nxm = ((Nx % 2 == 0) & (ixm == Nx // 2)).astype(np.int32)
+    nym = ((Ny % 2 == 0) & (iym == Ny // 2)).astype(np.int32)
+    M = len(sel)
+    f = np.ascontiguousarray(fin[sel])
+    fc = f.copy()
+    fc[..., 1] *= -1
+
+    def solve(ff, a, b, na, nb):
+        u, c = np.zeros_like(ff), np.zeros((M, 3, 4, 2))
+        corecheck.cc_bvp(Nz, D(Lz / 2), D(Lz), D(eta), GEOM[geometry], M,
+                         P(np.ascontiguousarray(a, dtype=np.float64)), P(np.ascontiguousarray(b, dtype=np.float64)),
+                         P(np.ascontiguousarray(na, dtype=np.int32)), P(np.ascontiguousarray(nb, dtype=np.int32)),
+                         P(ff), P(u), P(c))
+        return u[..., 0] + 1j * u[..., 1], c[..., 0] + 1j * c[..., 1]
+
+    uA, cA = solve(f, kx[sel], ky[sel], nyqx[sel], nyqy[sel])
+    uM, cM = solve(fc, kxm, kym, nxm, nym)
+    uB, cB = solve(f, -kxm, -kym, nxm, nym)
+    su, sc = np.abs(uA).max(), max(np.abs(cA).max(), 1e-300)
+    assert np.abs(np.conj(uM) - uA).max() <= 1e-13 * su
+    assert np.abs(uB - uA).max() <= 1e-13 * su
+    assert np.abs(np.conj(cM) - cB).max() <= 1e-13 * sc
+    if geometry != "nowall":
+        # the corrections themselves do differ (else the test pins nothing)
+        assert np.abs(cB - cA).max() > 1e-3 * sc
+
+
 SHAPES = [(17, 16, 2, 3, 0), (33, 16, 4, 1, 1), (61, 16, 4, 3, 0), (96, 16, 6, 1, 1), (96, 16, 6, 1, 0),
           (128, 16, 8, 1, 1), (150, 32, 8, 1, 1), (13, 4, 4, 3, 1)]
 
