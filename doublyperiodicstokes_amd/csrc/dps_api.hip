@@ -242,10 +242,11 @@ int build(dps_handle* h)
   h->RY = (wmax + 1 + h->TY - 1) / h->TY + ((h->Ny % h->TY) ? 1 : 0);
   if (std::min(2 * h->RX + 1, h->ntx) * std::min(2 * h->RY + 1, h->nty) > DPS_SPREAD_MAXT)
     return fail(DPS_ERR_ARG, "kernel support too wide for the spreading tiles");
-  // particle weight records: mono [wx][wy][z spread][z interp][z row];
-  // dipole [wx][wy][dx][dy][z spread][dz spread][z interp][dz interp][z row]
-  h->rs_m = h->km.wx + h->km.wy + 3 * h->km.wzmax;
-  h->rs_d = h->has_dip ? 2 * h->kd.wx + 2 * h->kd.wy + 5 * h->kd.wzmax : 0;
+  // particle weight records: mono [wx][wy][z spread][z interp]; dipole
+  // [wx][wy][dx][dy][z spread][dz spread][z interp][dz interp], z parts by
+  // interior row from sup.z
+  h->rs_m = h->km.wx + h->km.wy + 2 * h->km.wzmax;
+  h->rs_d = h->has_dip ? 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax : 0;
   {
     // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
     SpreadArgs S{};

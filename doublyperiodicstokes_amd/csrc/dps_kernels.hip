@@ -79,16 +79,24 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
       for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
       o += K.wy;
     }
-    // z support, wall images folded in: one entry per support node that
-    // reaches the interior (open-end ghosts dropped), with its interior row
-    // and the fold factor already applied to the spread and interpolation
-    // weights (class A factor for the value kernel, class B for d/dz)
+    // z support, wall images folded in and merged: dense weights of the
+    // interior rows klo .. klo + span - 1 (span <= W, since the direct rows
+    // and the folded images form one run of rows), each image added to the
+    // row it lands on with its fold factor (class A for the value kernel,
+    // class B for d/dz); open-end ghosts carry factor 0 and drop out
     const double a2 = K.alpha * K.alpha;
     const int W = K.wzmax;
     const int* kt = kind ? A.kt_d : A.kt_m;
     const double* fa = kind ? A.fA_d : A.fA_m;
     const double* fb = kind ? A.fB_d : nullptr;
-    int nq = 0, klo = 1 << 30, khi = -1;
+    int klo = 1 << 30, khi = -1;
+    for (int q = 0; q < s.nz; ++q) {
+      const int e = s.e0 + q;
+      if (fa[e] == 0.0 && (kind ? fb[e] : 0.0) == 0.0) continue;
+      klo = min(klo, kt[e]);
+      khi = max(khi, kt[e]);
+    }
+    for (int i = 0; i < (kind ? 4 : 2) * W; ++i) rec[o + i] = (T)0;
     for (int q = 0; q < s.nz; ++q) {
       const int e = s.e0 + q;
       const double sa = fa[e], sb = kind ? fb[e] : 0.0;
@@ -96,23 +104,20 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
       const double v = clamp_off(zext[e] - z, K.thresh);
       const double ph = es_phi(v, K.betaw, a2);
       const double qw = hxhy * zwe[e];
+      const int r = kt[e] - klo;
       if (kind == 0) {
-        rec[o + nq] = (T)(ph * sa);                 // spread
-        rec[o + W + nq] = (T)(ph * qw * sa);        // interpolation
-        rec[o + 2 * W + nq] = (T)kt[e];             // interior row
+        rec[o + r] += (T)(ph * sa);                 // spread
+        rec[o + W + r] += (T)(ph * qw * sa);        // interpolation
       } else {
         const double d = ph * es_dfac(v, K.betaw, a2);
-        rec[o + nq] = (T)(ph * sa);
-        rec[o + W + nq] = (T)(d * sb);
-        rec[o + 2 * W + nq] = (T)(ph * qw * sa);
-        rec[o + 3 * W + nq] = (T)(d * qw * sb);
-        rec[o + 4 * W + nq] = (T)kt[e];
+        rec[o + r] += (T)(ph * sa);
+        rec[o + W + r] += (T)(d * sb);
+        rec[o + 2 * W + r] += (T)(ph * qw * sa);
+        rec[o + 3 * W + r] += (T)(d * qw * sb);
       }
-      klo = min(klo, kt[e]);
-      khi = max(khi, kt[e]);
-      ++nq;
     }
-    sup.w = nq;
+    sup.z = (khi >= klo) ? klo : 0;
+    sup.w = (khi >= klo) ? khi - klo + 1 : 0;
     (kind ? A.sup_d : A.sup_m)[p] = sup;
     {
       int4 zr = A.zrange[p];
@@ -286,13 +291,15 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       __builtin_amdgcn_wave_barrier();
       for (int i = lane; i < rs; i += 64) st[i] = rec[i];
       __builtin_amdgcn_wave_barrier();
-      const int W = K.wzmax, nz = sup.w;
+      const int W = K.wzmax;
       const T* wxr = st;
       const T* wyr = st + K.wx;
       const T* dxr = wyr + K.wy;
       const T* dyr = dxr + K.wx;
-      const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);
-      const T* rowr = zr + (kind ? 4 : 2) * W;
+      const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);   // z weights of rows sup.z + q
+      // the entry's rows inside this chunk: q in [qa, qb)
+      const int qa = max(0, Z0 - sup.z), qb = min(sup.w, Z0 + nZ - sup.z);
+      if (qb <= qa) continue;
       T f0, f1, f2;
       if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
       else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
@@ -307,14 +314,13 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
           if (!cy) continue;
           const int ncol = cx * cy;
           const float incol = 1.0f / (float)ncol;
-          const int npts = ncol * nz;
+          const int npts = ncol * (qb - qa);
           for (int t = lane; t < npts; t += 64) {
             // x fastest, then y, then z
             int col, a;
-            const int q = fdiv(t, ncol, incol, &col);
+            const int q = qa + fdiv(t, ncol, incol, &col);
             const int b = fdiv(col, cx, icx, &a);
-            const int k = (int)rowr[q] - Z0;
-            if (k < 0 || k >= nZ) continue;
+            const int k = sup.z + q - Z0;
             T* dst = acc + k * PL + (ly0 + b) * RS + (lx0 + a);
             const int i = ix0 + a, j = iy0 + b;
             if (kind == 0) {
@@ -397,7 +403,6 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     const T* zr = kind ? (dyr + wy) : (wyr + wy);
     const T* wzi = zr + (kind ? 2 : 1) * W;   // interpolation weights (value kernel)
     const T* dzi = zr + 3 * W;                // d/dz kernel (dipole only)
-    const T* rowr = zr + (kind ? 4 : 2) * W;
     const int ncol = wx * wy;
     const int npts = ncol * nz;
     const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
@@ -412,7 +417,7 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
       int x = sup.x + a, y = sup.y + b;
       x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
       y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
-      const size_t idx = (size_t)(int)rowr[q] * zs + (size_t)y * A.Nx + x;
+      const size_t idx = (size_t)(sup.z + q) * zs + (size_t)y * A.Nx + x;
       const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
       if (kind == 0) {
         const T w = wxr[a] * wyr[b] * wzi[q];
