@@ -26,7 +26,8 @@ DPS_SOLVER_LANE = 1
 DPS_F32 = 0
 DPS_F64 = 1
 
-EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_mdot", "dps_stage_spread",
+EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions", "dps_mdot",
+           "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
            "dps_get_timings", "dps_get_precision", "dps_last_error")
 
@@ -66,6 +67,7 @@ def load():
     L.dps_create.argtypes = [ctypes.POINTER(DpsParams), ctypes.c_int, ctypes.POINTER(vp)]
     L.dps_destroy.argtypes = [vp]
     L.dps_set_positions.argtypes = [vp, vp, ctypes.c_int, vp]
+    L.dps_update_positions.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
     L.dps_mdot.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_stage_spread.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp]
     L.dps_stage_solve.argtypes = [vp, vp, vp, vp]

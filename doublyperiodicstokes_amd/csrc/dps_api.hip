@@ -749,6 +749,23 @@ int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* str
   return DPS_OK;
 }
 
+int dps_update_positions(dps_handle* h, void* positions, const void* velocities, double dt, int dtype,
+                         void* stream)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (!positions && h->N > 0) return fail(DPS_ERR_ARG, "null positions");
+  if (!(dt == dt)) return fail(DPS_ERR_ARG, "dt is not a number");
+  hipStream_t s = (hipStream_t)stream;
+  // the unit cell is [0, Lx] x [0, Ly] in x, y (as for dps_set_positions)
+  const double Lx = h->P.Lx, Ly = h->P.Ly;
+  if (dtype == DPS_F32) launch_advance<float>(positions, velocities, dt, h->N, 0.0, Lx, 0.0, Ly, s);
+  else launch_advance<double>(positions, velocities, dt, h->N, 0.0, Lx, 0.0, Ly, s);
+  HIPCHK(hipGetLastError());
+  return dps_set_positions(h, positions, dtype, stream);
+}
+
 int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* velocities, void* angular,
              int dtype, void* stream)
 {

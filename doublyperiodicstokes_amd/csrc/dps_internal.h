@@ -102,6 +102,9 @@ struct BvpArgs {
 
 template <class Tin, class T> void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s);
 void launch_reset_err(int* e, hipStream_t s);
+template <class Tp>
+void launch_advance(void* pos, const void* vel, double dt, int N, double xmin, double Lx, double ymin, double Ly,
+                    hipStream_t s);
 void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N, int ntiles,
                  hipStream_t s);
 template <class T, class Tin> void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s);

@@ -35,19 +35,51 @@ __device__ __forceinline__ int wrapi(int i, int n)
 // ---------------------------------------------------------------------------
 // positions -> stencils
 // ---------------------------------------------------------------------------
+// Periodic wrap of an offset from the cell origin into [0, L] as the
+// reference does it (ParticleList.cpp:561-580): images counted by truncation,
+// so x = L stays L.
+__device__ __forceinline__ double wrap_periodic(double x, double L)
+{
+  if (x < 0) {
+    const int img = 1 - ((int)(x / L));
+    x += img * L;
+  } else if (x > L) {
+    const int img = (int)(x / L);
+    x -= img * L;
+  }
+  return x;
+}
+
+// A step of positions kept on the device: x += dt u, then x and y wrapped
+// into the unit cell and written back in place (the reference's update
+// rewraps the caller's array, ParticleList.cpp:556-580).  z is left as is
+// for k_stencil to range-check.
+template <class Tp>
+__global__ void k_advance(Tp* __restrict__ pos, const Tp* __restrict__ vel, double dt, int N, double xmin,
+                          double Lx, double ymin, double Ly)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  double x = (double)pos[3 * p], y = (double)pos[3 * p + 1], z = (double)pos[3 * p + 2];
+  if (vel) {
+    x += dt * (double)vel[3 * p];
+    y += dt * (double)vel[3 * p + 1];
+    z += dt * (double)vel[3 * p + 2];
+  }
+  pos[3 * p] = (Tp)(wrap_periodic(x - xmin, Lx) + xmin);
+  pos[3 * p + 1] = (Tp)(wrap_periodic(y - ymin, Ly) + ymin);
+  pos[3 * p + 2] = (Tp)z;
+}
+
 template <class Tin, class T>
 __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
 {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= A.N) return;
-  double x = (double)pos_in[3 * p] - A.xmin;
-  double y = (double)pos_in[3 * p + 1] - A.ymin;
-  double z = (double)pos_in[3 * p + 2] - A.zmin;
   // periodic wrap in x, y (ParticleList.cpp:561-580)
-  if (x < 0) { int img = 1 - ((int)(x / A.Lx)); x += img * A.Lx; }
-  else if (x > A.Lx) { int img = (int)(x / A.Lx); x -= img * A.Lx; }
-  if (y < 0) { int img = 1 - ((int)(y / A.Ly)); y += img * A.Ly; }
-  else if (y > A.Ly) { int img = (int)(y / A.Ly); y -= img * A.Ly; }
+  double x = wrap_periodic((double)pos_in[3 * p] - A.xmin, A.Lx);
+  double y = wrap_periodic((double)pos_in[3 * p + 1] - A.ymin, A.Ly);
+  double z = (double)pos_in[3 * p + 2] - A.zmin;
   if (!(z >= 0 && z <= A.Lz)) {
     atomicOr(A.err, 1);
     z = z < 0 ? 0.0 : A.Lz;   // keep the stencil in range; the call reports an error
@@ -807,6 +839,17 @@ void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s)
   hipLaunchKernelGGL((k_stencil<Tin, T>), dim3((A.N + 127) / 128), dim3(128), 0, s, A,
                      (const Tin*)pos);
 }
+
+template <class Tp>
+void launch_advance(void* pos, const void* vel, double dt, int N, double xmin, double Lx, double ymin,
+                    double Ly, hipStream_t s)
+{
+  if (N == 0) return;
+  hipLaunchKernelGGL((k_advance<Tp>), dim3((N + 127) / 128), dim3(128), 0, s, (Tp*)pos, (const Tp*)vel, dt, N,
+                     xmin, Lx, ymin, Ly);
+}
+template void launch_advance<float>(void*, const void*, double, int, double, double, double, double, hipStream_t);
+template void launch_advance<double>(void*, const void*, double, int, double, double, double, double, hipStream_t);
 
 void launch_reset_err(int* e, hipStream_t s)
 {

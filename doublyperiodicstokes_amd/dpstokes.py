@@ -185,6 +185,26 @@ class DPStokes:
         N.check(N.load().dps_set_positions(self._h, ctypes.c_void_p(p.data_ptr()),
                                            _dtype_code(p.dtype), _stream()))
 
+    def update_positions_device(self, positions, velocities=None, dt=0.0):
+        """One position update kept on the device (ParticleList::update,
+        ParticleList.cpp:550-753): ``positions += dt * velocities`` in place on a
+        device tensor, x and y rewrapped into the unit cell, then relocated as
+        by setPositions.  ``velocities`` must have the positions' dtype."""
+        self._need()
+        torch = _torch()
+        if not (isinstance(positions, torch.Tensor) and positions.is_cuda and positions.is_contiguous()):
+            raise TypeError("positions must be a contiguous device tensor")
+        if positions.numel() != 3 * self.N or positions.dtype not in (torch.float32, torch.float64):
+            raise ValueError("positions must hold 3N float32 or float64 values")
+        v = None
+        if velocities is not None:
+            if not (isinstance(velocities, torch.Tensor) and velocities.is_cuda and velocities.is_contiguous()
+                    and velocities.dtype == positions.dtype and velocities.numel() == 3 * self.N):
+                raise TypeError("velocities must be a contiguous device tensor like positions")
+            v = ctypes.c_void_p(velocities.data_ptr())
+        N.check(N.load().dps_update_positions(self._h, ctypes.c_void_p(positions.data_ptr()), v,
+                                              float(dt), _dtype_code(positions.dtype), _stream()))
+
     def mdot_device(self, forces, torques=None, out_dtype=None):
         """Mdot on device tensors; returns (velocities, angular velocities or None)."""
         self._need()

@@ -92,6 +92,18 @@ int dps_destroy(dps_handle* h);
    Synchronises `stream` to report DPS_ERR_RANGE. */
 int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* stream);
 
+/* Position update of a time step, kept on the device (ParticleList::update,
+   ParticleList.cpp:550-753, which FCM.SetPositions calls from the second call
+   on, FCM.py:326-332): positions += dt * velocities (velocities may be NULL:
+   no move), x and y wrapped into the unit cell [0, Lx] x [0, Ly] by the
+   reference's rule and written back in place -- the reference rewraps the
+   caller's array too (ParticleList.cpp:556-558) -- then the particles are
+   located as by dps_set_positions.  A caller's integrator can loop Mdot and
+   this call with the positions never leaving the device.  Both arrays use
+   `dtype`; DPS_ERR_RANGE if a particle leaves the z extent. */
+int dps_update_positions(dps_handle* h, void* positions, const void* velocities, double dt, int dtype,
+                         void* stream);
+
 /* Mobility product (DPStokesGlue::Mdot, uammd_wrapper.cu:146-170; FCM.Mdot,
    FCM.py:334-441): velocities = M_tt F + M_tr T, angular = M_rt F + M_rr T.
    torques and angular may be NULL (then no torque is applied and angular
