@@ -232,8 +232,8 @@ int build(dps_handle* h)
       return fail(DPS_ERR_ARG, "nz is too large for the z-transform kernels");
   }
   // tiles for spreading
-  h->TX = std::min(16, h->Nx);
-  h->TY = std::min(8, h->Ny);
+  h->TX = SP_TX;
+  h->TY = SP_TY;
   h->ZC = std::min(h->prec == DPS_PREC_MIXED ? 32 : 16, h->Nz);
   h->ntx = (h->Nx + h->TX - 1) / h->TX;
   h->nty = (h->Ny + h->TY - 1) / h->TY;
@@ -251,6 +251,7 @@ int build(dps_handle* h)
     // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
     SpreadArgs S{};
     S.TX = h->TX; S.TY = h->TY; S.ZC = h->ZC; S.rs_max = std::max(h->rs_m, h->rs_d);
+    if (spread_npf(S.rs_max) == 0) return fail(DPS_ERR_ARG, "kernel z support too wide for the spreading records");
     while (spread_lds_bytes(S, h->tsz) > 65536 && S.ZC > 1) S.ZC /= 2;
     if (spread_lds_bytes(S, h->tsz) > 65536) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
     h->ZC = S.ZC;

@@ -6,6 +6,10 @@
 
 #define DPS_MAX_W 32          // largest x/y support (points) a stencil may have
 #define DPS_SPREAD_MAXT 128   // largest number of neighbour tiles a spreading workgroup scans
+#define DPS_SPREAD_BATCH 16   // queued spreading entries staged together
+// spreading tile: SP_TX x SP_TY x-y cells
+#define SP_TX 16
+#define SP_TY 8
 
 namespace dps {
 
@@ -61,6 +65,8 @@ struct SpreadArgs {
   void* grid;   // [3][Nz][Ny][Nx] grid precision
 };
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
+// words staged per thread and spreading entry for records of rs_max words (0: too long)
+int spread_npf(int rs_max);
 
 struct InterpArgs {
   int N, Nx, Ny, Nz;

@@ -237,36 +237,47 @@ __device__ __forceinline__ int fdiv(int t, int n, float inv, int* rem)
   return c;
 }
 
-// Spreading.  One workgroup owns a tile of TX x TY columns and ZC interior z
-// rows.  The particles of the tile's neighbourhood are culled in parallel (one
-// thread per candidate) into an LDS queue of (particle, kernel) entries whose
-// folded support reaches the tile; each wavefront then takes queue entries
-// and spreads every (column, z) point of the entry's clipped support with one
-// lane per point, accumulating with LDS atomics.  Every grid element of the
-// tile is written to HBM once, coalesced.
-template <class T, class Tin>
+// Spreading.  One workgroup owns a tile of SP_TX x SP_TY columns and ZC
+// interior z rows.  The particles of the tile's neighbourhood are culled in
+// parallel (one thread per candidate) into an LDS queue of (particle, kernel)
+// entries whose support reaches the tile.  The entries are staged in batches,
+// 16 threads per entry loading its weight record, support and force, one
+// batch ahead of the batch being spread, so a batch's global loads are in
+// flight together.  A wavefront spreads one entry at a time: each lane owns
+// one column of the entry's clipped support (column weights formed once) and
+// walks the entry's rows inside the chunk -- with at most 32 columns the wave
+// splits the rows into phases -- loading four rows' z weights ahead of their
+// LDS atomics.  The z weights are dense over the rows klo .. klo + span - 1
+// (folded images merged by the stencil).  Every grid element of the tile is
+// written to HBM once, coalesced.
+template <class T, class Tin, int NPF>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restrict__ F,
                                                 const Tin* __restrict__ Tq)
 {
   constexpr int MAXT = DPS_SPREAD_MAXT;   // neighbour tiles a workgroup scans (checked on the host)
-  // one dynamic LDS region, no static LDS: the accumulator must start at
-  // offset 0 so that FP64 accumulators are 8-byte aligned for ds_add_f64
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  T* acc = reinterpret_cast<T*>(smem_raw);
+  constexpr int B = DPS_SPREAD_BATCH;     // queued entries staged per batch, 16 threads each
+  constexpr int SLOT = 16 * NPF;          // words per staged entry: record, then header at rs_max
+  constexpr int TX = SP_TX, TY = SP_TY;
   // padded tile [c][z][y][x]: row stride TX+1, odd plane stride, so that the
   // lanes of one atomic instruction spread over the LDS banks
-  const int RS = A.TX + 1, PL = A.TY * RS + 1, vol = A.ZC * PL;
-  T* stage = acc + 3 * vol;                                       // [wave][rs] weight records
-  int* queue = reinterpret_cast<int*>(stage + (blockDim.x >> 6) * A.rs_max);   // p * 2 + kind
+  constexpr int RS = TX + 1, PL = TY * RS + 1;
+  // one dynamic LDS region, no static LDS: the accumulator starts at offset 0
+  // so that FP64 accumulators are 8-byte aligned for ds_add_f64
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  const int vol = A.ZC * PL;
+  T* bst = acc + 3 * vol;                                  // [B][SLOT] staged entries
+  int* queue = reinterpret_cast<int*>(bst + B * SLOT);     // p * 2 + kind
   int* tstart = queue + 2 * 256;   // first candidate of each neighbour tile
   int* tpre = tstart + MAXT;       // prefix sums of the tiles' counts
   int& qn = tpre[MAXT + 1];
   T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
   const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
-  const int X0 = tx * A.TX, Y0 = ty * A.TY, Z0 = zc * A.ZC;
-  const int nX = min(A.TX, A.Nx - X0), nY = min(A.TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
+  const int X0 = tx * TX, Y0 = ty * TY, Z0 = zc * A.ZC;
+  const int nX = min(TX, A.Nx - X0), nY = min(TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
   for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (T)0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
   const int nkind = Tq ? 2 : 1;
   const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
   const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
@@ -286,7 +297,27 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   }
   __syncthreads();
   const int ncand = tpre[ntl];
-  T* st = stage + wave * A.rs_max;
+  const int HO = A.rs_max;   // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2
+  // staging thread (ss, su) loads words su, su+16, ... of entry ss of a batch
+  const int ss = threadIdx.x >> 4, su = threadIdx.x & 15;
+  auto prefetch = [&](int qb, int nb, T (&v)[NPF]) {
+    if (ss >= nb) return;
+    const int ent = queue[qb + ss];
+    const int p = ent >> 1, kind = ent & 1;
+    const int rs = kind ? A.rs_d : A.rs_m;
+    const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * rs;
+    const int* sp = reinterpret_cast<const int*>((kind ? A.sup_d : A.sup_m) + p);
+    const Tin* fp = (kind ? Tq : F) + 3 * (size_t)p;
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int off = su + 16 * k;
+      T xv = (T)0;
+      if (off < rs) xv = rec[off];
+      else if (off >= HO && off < HO + 4) xv = (T)sp[off - HO];
+      else if (off >= HO + 4 && off < HO + 7) xv = (T)fp[off - HO - 4];
+      v[k] = xv;
+    }
+  };
   for (int c0 = 0; c0 < ncand; c0 += blockDim.x) {
     // ---- cull: one candidate particle per thread ----
     const int ci = c0 + threadIdx.x;
@@ -310,88 +341,131 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       }
     }
     __syncthreads();
-    // ---- spread the queued entries, one wavefront per entry ----
+    // ---- spread the queued entries in staged batches, one wavefront per entry ----
     const int nq_entries = qn;
-    for (int qi = wave; qi < nq_entries; qi += nwaves) {
-      const int ent = queue[qi];
-      const int p = ent >> 1, kind = ent & 1;
-      const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
-      const KernelDesc& K = kind ? A.kd : A.km;
-      const int rs = kind ? A.rs_d : A.rs_m;
-      // the entry's weight record into this wave's LDS slot (one coalesced read)
-      const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * rs;
-      __builtin_amdgcn_wave_barrier();
-      for (int i = lane; i < rs; i += 64) st[i] = rec[i];
-      __builtin_amdgcn_wave_barrier();
-      const int W = K.wzmax;
-      const T* wxr = st;
-      const T* wyr = st + K.wx;
-      const T* dxr = wyr + K.wy;
-      const T* dyr = dxr + K.wx;
-      const T* zr = kind ? (dyr + K.wy) : (wyr + K.wy);   // z weights of rows sup.z + q
-      // the entry's rows inside this chunk: q in [qa, qb)
-      const int qa = max(0, Z0 - sup.z), qb = min(sup.w, Z0 + nZ - sup.z);
-      if (qb <= qa) continue;
-      T f0, f1, f2;
-      if (kind == 0) { f0 = (T)F[3 * p]; f1 = (T)F[3 * p + 1]; f2 = (T)F[3 * p + 2]; }
-      else { f0 = (T)Tq[3 * p]; f1 = (T)Tq[3 * p + 1]; f2 = (T)Tq[3 * p + 2]; }
-      for (int sxp = -1; sxp <= 1; ++sxp) {
-        int ix0, lx0;
-        const int cx = clip_piece(sup.x, K.wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
-        if (!cx) continue;
-        const float icx = 1.0f / (float)cx;
-        for (int syp = -1; syp <= 1; ++syp) {
-          int iy0, ly0;
-          const int cy = clip_piece(sup.y, K.wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
-          if (!cy) continue;
-          const int ncol = cx * cy;
-          const float incol = 1.0f / (float)ncol;
-          const int npts = ncol * (qb - qa);
-          for (int t = lane; t < npts; t += 64) {
-            // x fastest, then y, then z
-            int col, a;
-            const int q = qa + fdiv(t, ncol, incol, &col);
-            const int b = fdiv(col, cx, icx, &a);
-            const int k = sup.z + q - Z0;
-            T* dst = acc + k * PL + (ly0 + b) * RS + (lx0 + a);
-            const int i = ix0 + a, j = iy0 + b;
-            if (kind == 0) {
-              const T w = wxr[i] * wyr[j] * zr[q];
-              atomicAdd(dst, w * f0);
-              atomicAdd(dst + vol, w * f1);
-              atomicAdd(dst + 2 * vol, w * f2);
-            } else {
-              const T wzA = zr[q], dzB = zr[W + q];
-              const T gx = dxr[i] * wyr[j] * wzA;
-              const T gy = wxr[i] * dyr[j] * wzA;
-              const T gz = wxr[i] * wyr[j] * dzB;
-              const T h = (T)0.5;
-              atomicAdd(dst, h * (gy * f2 - gz * f1));
-              atomicAdd(dst + vol, h * (gz * f0 - gx * f2));
-              atomicAdd(dst + 2 * vol, h * (gx * f1 - gy * f0));
+    T pf[NPF];
+    prefetch(0, min(B, nq_entries), pf);
+    for (int qb = 0; qb < nq_entries; qb += B) {
+      const int nb = min(B, nq_entries - qb);
+      if (ss < nb)
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) bst[ss * SLOT + su + 16 * k] = pf[k];
+      __syncthreads();
+      if (qb + B < nq_entries) prefetch(qb + B, min(B, nq_entries - qb - B), pf);
+      for (int si = wave; si < nb; si += nwaves) {
+        const T* st = bst + si * SLOT;
+        // the entry's header is the same for every lane: scalar registers
+        const int kind = __builtin_amdgcn_readfirstlane(queue[qb + si] & 1);
+        const int bx = __builtin_amdgcn_readfirstlane((int)st[HO]);
+        const int by = __builtin_amdgcn_readfirstlane((int)st[HO + 1]);
+        const int klo = __builtin_amdgcn_readfirstlane((int)st[HO + 2]);
+        const int span = __builtin_amdgcn_readfirstlane((int)st[HO + 3]);
+        // the entry's rows inside this chunk: q in [qa, qb2)
+        const int qa = max(0, Z0 - klo), qb2 = min(span, Z0 + nZ - klo);
+        if (qb2 <= qa) continue;
+        const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
+        const int W = kind ? A.kd.wzmax : A.km.wzmax;
+        const T* wxr = st;
+        const T* wyr = st + wx;
+        const T* dxr = wyr + wy;
+        const T* dyr = dxr + wx;
+        const T* zr = kind ? (dyr + wy) : (wyr + wy);   // z weights of rows klo + q
+        const T f0 = st[HO + 4], f1 = st[HO + 5], f2 = st[HO + 6];
+        for (int sxp = -1; sxp <= 1; ++sxp) {
+          int ix0, lx0;
+          const int cx = clip_piece(bx, wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
+          if (!cx) continue;
+          for (int syp = -1; syp <= 1; ++syp) {
+            int iy0, ly0;
+            const int cy = clip_piece(by, wy, Y0, nY, A.Ny, syp * A.Ny, &iy0, &ly0);
+            if (!cy) continue;
+            // lane -> one support column (x fastest); with at most 32 columns
+            // the wave splits the rows into P phases
+            const int L = cx * cy;
+            const int P = (L <= 32) ? 64 / L : 1;
+            for (int cb = 0; cb < L; cb += (P > 1) ? L : 64) {
+              int phase = 0, col = cb + lane;
+              if (P > 1) {
+                phase = lane / L;
+                col = lane - phase * L;
+              }
+              if (phase >= P || col >= L) continue;
+              const int b = col / cx, a = col - b * cx;
+              const int i = ix0 + a, j = iy0 + b;
+              T* dcol = acc + (klo - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row klo + q at + q PL
+              const int q1 = qb2;
+              if (kind == 0) {
+                const T wxy = wxr[i] * wyr[j];
+                const T g0 = f0, g1 = f1, g2 = f2;
+                int q = qa + phase;
+                for (; q + 3 * P < q1; q += 4 * P) {
+                  const T z0 = zr[q], z1 = zr[q + P], z2 = zr[q + 2 * P], z3 = zr[q + 3 * P];
+                  const T w0 = wxy * z0, w1 = wxy * z1, w2 = wxy * z2, w3 = wxy * z3;
+                  T* d0 = dcol + q * PL;
+                  atomicAdd(d0, w0 * g0); atomicAdd(d0 + vol, w0 * g1); atomicAdd(d0 + 2 * vol, w0 * g2);
+                  T* d1 = d0 + P * PL;
+                  atomicAdd(d1, w1 * g0); atomicAdd(d1 + vol, w1 * g1); atomicAdd(d1 + 2 * vol, w1 * g2);
+                  T* d2 = d1 + P * PL;
+                  atomicAdd(d2, w2 * g0); atomicAdd(d2 + vol, w2 * g1); atomicAdd(d2 + 2 * vol, w2 * g2);
+                  T* d3 = d2 + P * PL;
+                  atomicAdd(d3, w3 * g0); atomicAdd(d3 + vol, w3 * g1); atomicAdd(d3 + 2 * vol, w3 * g2);
+                }
+                for (; q < q1; q += P) {
+                  const T w = wxy * zr[q];
+                  T* d = dcol + q * PL;
+                  atomicAdd(d, w * g0); atomicAdd(d + vol, w * g1); atomicAdd(d + 2 * vol, w * g2);
+                }
+              } else {
+                const T h = (T)0.5;
+                const T ax = dxr[i] * wyr[j], ay = wxr[i] * dyr[j], az = wxr[i] * wyr[j];
+                int q = qa + phase;
+                for (; q + P < q1; q += 2 * P) {
+                  const T zA0 = zr[q], zB0 = zr[W + q], zA1 = zr[q + P], zB1 = zr[W + q + P];
+                  {
+                    const T gx = ax * zA0, gy = ay * zA0, gz = az * zB0;
+                    T* d = dcol + q * PL;
+                    atomicAdd(d, h * (gy * f2 - gz * f1));
+                    atomicAdd(d + vol, h * (gz * f0 - gx * f2));
+                    atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                  }
+                  {
+                    const T gx = ax * zA1, gy = ay * zA1, gz = az * zB1;
+                    T* d = dcol + (q + P) * PL;
+                    atomicAdd(d, h * (gy * f2 - gz * f1));
+                    atomicAdd(d + vol, h * (gz * f0 - gx * f2));
+                    atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                  }
+                }
+                for (; q < q1; q += P) {
+                  const T zA = zr[q], zB = zr[W + q];
+                  const T gx = ax * zA, gy = ay * zA, gz = az * zB;
+                  T* d = dcol + q * PL;
+                  atomicAdd(d, h * (gy * f2 - gz * f1));
+                  atomicAdd(d + vol, h * (gz * f0 - gx * f2));
+                  atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                }
+              }
             }
           }
         }
       }
+      __syncthreads();   // the batch's slots are free again
     }
-    __syncthreads();
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
   }
   // one write per grid element of the tile: thread -> fixed x, rows (z, y)
-  // in steps of blockDim / TX (one float-reciprocal division per element)
+  // in steps of blockDim / TX
   {
-    const int ix = threadIdx.x % A.TX, rstep = blockDim.x / A.TX;
-    const int nrow = nZ * nY;
-    const float iny = 1.0f / (float)nY;
-    if (ix < nX && threadIdx.x < rstep * A.TX)
+    const int ix = threadIdx.x % TX, rstep = blockDim.x / TX;
+    const int nrow = nZ * TY;
+    if (ix < nX)
       for (int c = 0; c < 3; ++c) {
         T* gc = grid + ((size_t)c * A.Nz + Z0) * A.Ny * A.Nx + (size_t)Y0 * A.Nx + X0 + ix;
         const T* ac = acc + c * vol + ix;
-        for (int row = threadIdx.x / A.TX; row < nrow; row += rstep) {
-          int jy;
-          const int kz = fdiv(row, nY, iny, &jy);
-          gc[((size_t)kz * A.Ny + jy) * A.Nx] = ac[kz * PL + jy * RS];
+        for (int row = threadIdx.x / TX; row < nrow; row += rstep) {
+          const int kz = row / TY, jy = row % TY;
+          if (jy < nY) gc[((size_t)kz * A.Ny + jy) * A.Nx] = ac[kz * PL + jy * RS];
         }
       }
   }
@@ -868,10 +942,19 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
                        order, N);
 }
 
+int spread_npf(int rs_max)
+{
+  // staged words per entry: record + 7 header words, 16 threads per entry
+  static const int sizes[] = {8, 10, 12, 16, 20, 24, 32};
+  for (int npf : sizes)
+    if (rs_max + 7 <= 16 * npf) return npf;
+  return 0;
+}
+
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz)
 {
-  // accumulator, per-wave record slots (4 waves), queue, tile table + counter
-  return ((size_t)3 * A.ZC * (A.TY * (A.TX + 1) + 1) + 4 * (size_t)A.rs_max) * tsz +
+  // accumulator, staged batch, queue, tile table + counter
+  return ((size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) + (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max)) * tsz +
          (2 * 256 + 2 * DPS_SPREAD_MAXT + 2) * sizeof(int);
 }
 
@@ -880,8 +963,14 @@ void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream
 {
   const size_t lds = spread_lds_bytes(A, sizeof(T));
   dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
-  hipLaunchKernelGGL((k_spread<T, Tin>), grid, dim3(256), lds, s, A, (const Tin*)F,
-                     (const Tin*)Tq);
+  const Tin *f = (const Tin*)F, *t = (const Tin*)Tq;
+  switch (spread_npf(A.rs_max)) {
+#define SPREAD_CASE(NPF) \
+  case NPF: hipLaunchKernelGGL((k_spread<T, Tin, NPF>), grid, dim3(256), lds, s, A, f, t); break;
+    SPREAD_CASE(8) SPREAD_CASE(10) SPREAD_CASE(12) SPREAD_CASE(16) SPREAD_CASE(20) SPREAD_CASE(24)
+    default: hipLaunchKernelGGL((k_spread<T, Tin, 32>), grid, dim3(256), lds, s, A, f, t); break;
+#undef SPREAD_CASE
+  }
 }
 
 template <class T, class Tout>
