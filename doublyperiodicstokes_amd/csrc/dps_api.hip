@@ -234,9 +234,9 @@ int build(dps_handle* h)
   // tiles for spreading
   h->TX = SP_TX;
   h->TY = SP_TY;
-  // 16 rows per z chunk: more, smaller workgroups beat longer chunks (fewer
-  // entries replicated across chunks) on the C2 workload (8: 213 us, 12: 203,
-  // 16: 199, 32: 247)
+  // 16 rows per z chunk: more, smaller workgroups beat longer chunks with
+  // fewer entries replicated across chunks (slit, N=16384, 256x256x96 grid:
+  // 8 rows 213 us, 12: 203, 16: 199, 32: 247)
   h->ZC = std::min(16, h->Nz);
   h->ntx = (h->Nx + h->TX - 1) / h->TX;
   h->nty = (h->Ny + h->TY - 1) / h->TY;
