@@ -19,6 +19,7 @@ DPS_ERR_ALLOC = -6
 DPS_MODE_NOWALL = 1
 DPS_MODE_BOTTOM = 2
 DPS_MODE_SLIT = 3
+DPS_MODE_PERIODIC = 4
 DPS_PREC_DOUBLE = 0
 DPS_PREC_MIXED = 1
 DPS_SOLVER_AUTO = 0

@@ -53,7 +53,7 @@ struct dps_handle {
   dps_params P;
   int N = 0;
   int geometry = 0, prec = 0;
-  double hx = 0, hy = 0, Lz = 0, H = 0;
+  double hx = 0, hy = 0, Lz = 0, H = 0, hz = 0;   // hz: periodic z spacing
   int Nx = 0, Ny = 0, Nz = 0, Nkx = 0, Nm = 0;
   KernelDesc km{}, kd{};
   bool has_dip = false;
@@ -165,6 +165,26 @@ int setup_kernel(dps_handle* h, int w, double alpha, double beta, bool dipole, K
   return DPS_OK;
 }
 
+int build_tiles(dps_handle* h, int wmax);
+
+// Triply periodic: uniform periodic z nodes k hz, hz = Lz / Nz (FCM.py:182-186,
+// GridAndKernelConfig.py:393-417); z supports like x and y; a 3-D FFT and the
+// Fourier solve replace the Chebyshev machinery.
+int build_periodic(dps_handle* h, int wmax)
+{
+  h->hz = h->Lz / h->Nz;
+  for (int k = 0; k < (h->has_dip ? 2 : 1); ++k) {
+    KernelDesc* K = k ? &h->kd : &h->km;
+    K->wzmax = (int)std::lround(2 * K->thresh / h->hz);
+    K->nzeff = h->Nz;
+    K->ext_up = 0;
+    if (K->wzmax < 1 || K->wzmax > DPS_MAX_W || K->wzmax > h->Nz)
+      return fail(DPS_ERR_ARG, "kernel support in z is out of range for the periodic z grid");
+  }
+  h->fused = false;
+  return build_tiles(h, wmax);
+}
+
 int build(dps_handle* h)
 {
   const dps_params& P = h->P;
@@ -190,6 +210,7 @@ int build(dps_handle* h)
   const int wmax = std::max(std::max(h->km.wx, h->km.wy), h->has_dip ? std::max(h->kd.wx, h->kd.wy) : 0);
   if (h->Nx < 2 * wmax || h->Ny < 2 * wmax)
     return fail(DPS_ERR_ARG, "nx and ny must be at least twice the kernel support");
+  if (h->geometry == DPS_MODE_PERIODIC) return build_periodic(h, wmax);
   // z grid: Chebyshev nodes on [0, Lz] (FCM.py:220)
   clencurt(h->zpts, h->zwts, 0, h->Lz, h->Nz);
   const bool wall_top = (h->geometry == DPS_MODE_SLIT);
@@ -231,6 +252,12 @@ int build(dps_handle* h)
     if (z_lds_bytes(Z, h->prec == DPS_PREC_MIXED) > 160 * 1024)
       return fail(DPS_ERR_ARG, "nz is too large for the z-transform kernels");
   }
+  return build_tiles(h, wmax);
+}
+
+// spreading tiles and particle records (both z grids)
+int build_tiles(dps_handle* h, int wmax)
+{
   // tiles for spreading
   h->TX = SP_TX;
   h->TY = SP_TY;
@@ -297,6 +324,9 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc((unsigned char**)&h->d_grid, ng * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specA, nspec * 2 * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specB, nspec * 2 * ts))) return rc;
+  // triply periodic: the r2c spectrum [c][kz][ky][kx] has the same size, and
+  // nothing else is needed
+  if (h->geometry == DPS_MODE_PERIODIC) return DPS_OK;
   // z tables
   if ((rc = h->upload(&h->d_zext_m, h->zm.z))) return rc;
   if ((rc = h->upload(&h->d_zw_m, h->zm.w))) return rc;
@@ -437,13 +467,16 @@ int allocate_fused(dps_handle* h)
 int make_plans(dps_handle* h)
 {
   int rc;
-  // rocFFT plans: 2-D over (x, y), batched over (component, z)
+  // rocFFT plans: 2-D over (x, y), batched over (component, z); triply
+  // periodic: 3-D over (x, y, z), batched over the components
   const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
-  size_t lengths[2] = {(size_t)h->Nx, (size_t)h->Ny};
+  const bool tp = h->geometry == DPS_MODE_PERIODIC;
+  size_t lengths[3] = {(size_t)h->Nx, (size_t)h->Ny, (size_t)h->Nz};
+  const size_t dim = tp ? 3 : 2, batch = tp ? 3 : (size_t)3 * h->Nz;
   FFTCHK(rocfft_plan_create(&h->plan_fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                            fp, 2, lengths, (size_t)3 * h->Nz, nullptr));
+                            fp, dim, lengths, batch, nullptr));
   FFTCHK(rocfft_plan_create(&h->plan_inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                            fp, 2, lengths, (size_t)3 * h->Nz, nullptr));
+                            fp, dim, lengths, batch, nullptr));
   size_t w1 = 0, w2 = 0;
   FFTCHK(rocfft_plan_get_work_buffer_size(h->plan_fwd, &w1));
   FFTCHK(rocfft_plan_get_work_buffer_size(h->plan_inv, &w2));
@@ -505,6 +538,8 @@ StencilArgs stencil_args(dps_handle* h)
   A.fA_d = h->d_fA_d;
   A.fB_d = h->d_fB_d;
   A.zrange = h->d_zrange;
+  A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
+  A.hz = h->hz;
   return A;
 }
 
@@ -520,6 +555,7 @@ SpreadArgs spread_args(dps_handle* h)
   A.rec_m = h->d_rec_m; A.rec_d = h->d_rec_d; A.rs_m = h->rs_m; A.rs_d = h->rs_d;
   A.rs_max = std::max(h->rs_m, h->rs_d);
   A.grid = h->d_grid;
+  A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
   return A;
 }
 
@@ -534,6 +570,7 @@ InterpArgs interp_args(dps_handle* h)
   A.kt_m = h->d_kt_m; A.kt_d = h->d_kt_d;
   A.fA_m = h->d_fA_m; A.fA_d = h->d_fA_d; A.fB_d = h->d_fB_d;
   A.grid = h->d_grid;
+  A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
   return A;
 }
 
@@ -616,6 +653,18 @@ int do_solve(dps_handle* h, hipStream_t s, int ev0)
   int rc;
   if ((rc = run_fft(h, true, h->d_grid, h->d_specA, s))) return rc;
   mark(h, ev0 + 1, s);
+  if (h->geometry == DPS_MODE_PERIODIC) {
+    // Fourier solve (reported as the bvp stage; no z transforms)
+    if (mixed(h)) launch_tp_solve<float>(h->d_specA, h->d_specB, h->Nx, h->Ny, h->Nz, h->P.Lx, h->P.Ly, h->Lz,
+                                         h->P.viscosity, s);
+    else launch_tp_solve<double>(h->d_specA, h->d_specB, h->Nx, h->Ny, h->Nz, h->P.Lx, h->P.Ly, h->Lz,
+                                 h->P.viscosity, s);
+    HIPCHK(hipGetLastError());
+    mark(h, ev0 + 3, s);
+    if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+    mark(h, ev0 + 5, s);
+    return DPS_OK;
+  }
   if (h->fused) {
     // z transforms are inside the fused launch: no events of their own
     // (dps_get_timings reports them as zero)
@@ -681,8 +730,9 @@ int dps_create(const dps_params* params, int number_particles, dps_handle** out)
   *out = nullptr;
   if (!params) return fail(DPS_ERR_ARG, "null parameters");
   const dps_params& P = *params;
-  if (P.mode != DPS_MODE_NOWALL && P.mode != DPS_MODE_BOTTOM && P.mode != DPS_MODE_SLIT)
-    return fail(DPS_ERR_ARG, "mode must be nowall, bottom or slit");
+  if (P.mode != DPS_MODE_NOWALL && P.mode != DPS_MODE_BOTTOM && P.mode != DPS_MODE_SLIT &&
+      P.mode != DPS_MODE_PERIODIC)
+    return fail(DPS_ERR_ARG, "mode must be nowall, bottom, slit or periodic");
   if (P.precision != DPS_PREC_DOUBLE && P.precision != DPS_PREC_MIXED)
     return fail(DPS_ERR_ARG, "precision must be DPS_PREC_DOUBLE or DPS_PREC_MIXED");
   if (P.solver != DPS_SOLVER_AUTO && P.solver != DPS_SOLVER_LANE)
@@ -874,7 +924,7 @@ int dps_get_timings(dps_handle* h, double* out)
   out[0] = 0.0;   // locate/sort happens in dps_set_positions
   for (int i = 0; i < 7; ++i) {
     int a = i, b = i + 1;
-    if (h->fused) {
+    if (h->fused || h->geometry == DPS_MODE_PERIODIC) {
       if (i == 2 || i == 4) {
         out[i + 1] = 0.0;
         continue;

@@ -97,17 +97,22 @@ struct Stencil {
   int bx, by, e0, nz;
 };
 
+// First support point of a uniform direction: xclose = (int)(x/h), +1 for odd
+// w past the half cell, start = xclose - w/2 + even (ParticleList.cpp:337-347)
+DPS_HD int support_start(double x, double h, int w)
+{
+  const int even = -1 * (w % 2) + 1;
+  int c = (int)(x / h);
+  c += ((w % 2) && (x / h - c > 1.0 / 2.0) ? 1 : 0);
+  return c - w / 2 + even;
+}
+
 DPS_HD Stencil stencil_support(double x, double y, double z, double hx, double hy,
                                const KernelDesc& k, const double* zext)
 {
   Stencil s;
-  const int wx = k.wx, wy = k.wy;
-  const int evenx = -1 * (wx % 2) + 1, eveny = -1 * (wy % 2) + 1;
-  int xc = (int)(x / hx), yc = (int)(y / hy);
-  xc += ((wx % 2) && (x / hx - xc > 1.0 / 2.0) ? 1 : 0);
-  yc += ((wy % 2) && (y / hy - yc > 1.0 / 2.0) ? 1 : 0);
-  s.bx = xc - wx / 2 + evenx;
-  s.by = yc - wy / 2 + eveny;
+  s.bx = support_start(x, hx, k.wx);
+  s.by = support_start(y, hy, k.wy);
   const int high = lower_bound_desc(zext, k.nzeff, z - k.thresh);
   const int low = lower_bound_desc(zext, k.nzeff, z + k.thresh);
   int ir = high;

@@ -51,10 +51,13 @@ struct StencilArgs {
   const int* kt_d;
   const double *fA_m, *fA_d, *fB_d;
   int4* zrange;          // per particle: (klo, khi) monopole, (klo, khi) dipole
+  int periodic_z;        // triply periodic: uniform z nodes k hz, wrapped like x and y
+  double hz;
 };
 
 struct SpreadArgs {
   int Nx, Ny, Nz;
+  int periodic_z;   // rows of a support wrap modulo Nz
   int TX, TY, ZC, ntx, nty, RX, RY;
   const int *start, *order;
   const int4* zrange;
@@ -78,7 +81,14 @@ struct InterpArgs {
   const int *kt_m, *kt_d;
   const double *fA_m, *fA_d, *fB_d;
   const void* grid;
+  int periodic_z;   // rows of a support wrap modulo Nz
 };
+
+// Fourier solve of the triply periodic Stokes problem on the r2c spectrum
+// [c][kz][ky][kx] (kx in [0, Nx/2]), scaled by 1/(Nx Ny Nz) for the c2r
+template <class T>
+void launch_tp_solve(const void* in, void* out, int Nx, int Ny, int Nz, double Lx, double Ly, double Lz,
+                     double eta, hipStream_t s);
 
 struct ZArgs {
   int N, NH, NE, NO;   // nodes, node pairs, even/odd coefficient counts

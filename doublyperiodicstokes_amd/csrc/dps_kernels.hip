@@ -80,6 +80,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
   double x = wrap_periodic((double)pos_in[3 * p] - A.xmin, A.Lx);
   double y = wrap_periodic((double)pos_in[3 * p + 1] - A.ymin, A.Ly);
   double z = (double)pos_in[3 * p + 2] - A.zmin;
+  if (A.periodic_z) z = wrap_periodic(z, A.Lz);   // ParticleList.cpp:583-595
   if (!(z >= 0 && z <= A.Lz)) {
     atomicOr(A.err, 1);
     z = z < 0 ? 0.0 : A.Lz;   // keep the stencil in range; the call reports an error
@@ -92,8 +93,16 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
     const KernelDesc& K = kind ? A.kd : A.km;
     const double* zext = kind ? A.zext_d : A.zext_m;
     const double* zwe = kind ? A.zw_d : A.zw_m;
-    Stencil s = stencil_support(x, y, z, A.hx, A.hy, K, zext);
-    if (s.nz > K.wzmax) s.nz = K.wzmax;   // cannot happen (wzmax bounds the window)
+    Stencil s;
+    if (A.periodic_z) {
+      s.bx = support_start(x, A.hx, K.wx);
+      s.by = support_start(y, A.hy, K.wy);
+      s.e0 = 0;
+      s.nz = 0;
+    } else {
+      s = stencil_support(x, y, z, A.hx, A.hy, K, zext);
+      if (s.nz > K.wzmax) s.nz = K.wzmax;   // cannot happen (wzmax bounds the window)
+    }
     int4 sup;
     sup.x = s.bx; sup.y = s.by; sup.z = s.e0; sup.w = s.nz;
     T* rec = (T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
@@ -110,6 +119,41 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
       o += K.wx;
       for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
       o += K.wy;
+    }
+    if (A.periodic_z) {
+      // uniform periodic z (ParticleList.cpp:361-368): w_z = W points from
+      // support_start, rows wrapped modulo Nz where used; interpolation
+      // weight hx hy hz
+      const double a2 = K.alpha * K.alpha;
+      const int W = K.wzmax;
+      const int bz = support_start(z, A.hz, W);
+      for (int q = 0; q < W; ++q) {
+        const double v = clamp_off((double)(bz + q) * A.hz - z, K.thresh);
+        const double ph = es_phi(v, K.betaw, a2);
+        const double qw = hxhy * A.hz;
+        if (kind == 0) {
+          rec[o + q] = (T)ph;
+          rec[o + W + q] = (T)(ph * qw);
+        } else {
+          const double d = ph * es_dfac(v, K.betaw, a2);
+          rec[o + q] = (T)ph;
+          rec[o + W + q] = (T)d;
+          rec[o + 2 * W + q] = (T)(ph * qw);
+          rec[o + 3 * W + q] = (T)(d * qw);
+        }
+      }
+      sup.z = bz;
+      sup.w = W;
+      (kind ? A.sup_d : A.sup_m)[p] = sup;
+      int4 zr = A.zrange[p];
+      if (kind) { zr.z = bz; zr.w = bz + W - 1; }
+      else { zr.x = bz; zr.y = bz + W - 1; zr.z = 1 << 30; zr.w = -1; }
+      A.zrange[p] = zr;
+      if (kind == 0) {
+        const int cx = wrapi(s.bx + K.wx / 2, A.Nx), cy = wrapi(s.by + K.wy / 2, A.Ny);
+        A.key[p] = (cy / A.TY) * A.ntx + (cx / A.TX);
+      }
+      continue;
     }
     // z support, wall images folded in and merged: dense weights of the
     // interior rows klo .. klo + span - 1 (span <= W, since the direct rows
@@ -328,7 +372,11 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       const int4 zr = A.zrange[p];
       for (int kind = 0; kind < nkind; ++kind) {
         const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
-        if (khi < Z0 || klo >= Z0 + nZ) continue;
+        // rows klo..khi, and their periodic images when z is periodic
+        bool hz = false;
+        for (int sh = A.periodic_z ? -1 : 0; sh <= (A.periodic_z ? 1 : 0); ++sh)
+          hz = hz || !(khi + sh * A.Nz < Z0 || klo + sh * A.Nz >= Z0 + nZ);
+        if (!hz) continue;
         const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
         const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
         bool hx = false, hy = false;
@@ -360,9 +408,6 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
         const int by = __builtin_amdgcn_readfirstlane((int)st[HO + 1]);
         const int klo = __builtin_amdgcn_readfirstlane((int)st[HO + 2]);
         const int span = __builtin_amdgcn_readfirstlane((int)st[HO + 3]);
-        // the entry's rows inside this chunk: q in [qa, qb2)
-        const int qa = max(0, Z0 - klo), qb2 = min(span, Z0 + nZ - klo);
-        if (qb2 <= qa) continue;
         const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
         const int W = kind ? A.kd.wzmax : A.km.wzmax;
         const T* wxr = st;
@@ -371,6 +416,12 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
         const T* dyr = dxr + wx;
         const T* zr = kind ? (dyr + wy) : (wyr + wy);   // z weights of rows klo + q
         const T f0 = st[HO + 4], f1 = st[HO + 5], f2 = st[HO + 6];
+        // rows klo + q (+ a period when z is periodic) inside this chunk:
+        // q in [qa, qb2)
+        for (int szp = A.periodic_z ? -1 : 0; szp <= (A.periodic_z ? 1 : 0); ++szp) {
+        const int kls = klo + szp * A.Nz;
+        const int qa = max(0, Z0 - kls), qb2 = min(span, Z0 + nZ - kls);
+        if (qb2 <= qa) continue;
         for (int sxp = -1; sxp <= 1; ++sxp) {
           int ix0, lx0;
           const int cx = clip_piece(bx, wx, X0, nX, A.Nx, sxp * A.Nx, &ix0, &lx0);
@@ -392,7 +443,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               if (phase >= P || col >= L) continue;
               const int b = col / cx, a = col - b * cx;
               const int i = ix0 + a, j = iy0 + b;
-              T* dcol = acc + (klo - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row klo + q at + q PL
+              T* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
               const int q1 = qb2;
               if (kind == 0) {
                 const T wxy = wxr[i] * wyr[j];
@@ -447,6 +498,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               }
             }
           }
+        }
         }
       }
       __syncthreads();   // the batch's slots are free again
@@ -523,7 +575,9 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
       int x = sup.x + a, y = sup.y + b;
       x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
       y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
-      const size_t idx = (size_t)(sup.z + q) * zs + (size_t)y * A.Nx + x;
+      int zr_ = sup.z + q;
+      if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+      const size_t idx = (size_t)zr_ * zs + (size_t)y * A.Nx + x;
       const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
       if (kind == 0) {
         const T w = wxr[a] * wyr[b] * wzi[q];
@@ -924,6 +978,69 @@ void launch_advance(void* pos, const void* vel, double dt, int N, double xmin, d
 }
 template void launch_advance<float>(void*, const void*, double, int, double, double, double, double, hipStream_t);
 template void launch_advance<double>(void*, const void*, double, int, double, double, double, double, hipStream_t);
+
+// Triply periodic Fourier solve (TPTools.cpp:6-46, Solvers.py:197-255), one
+// thread per r2c mode: with D = i k (the unpaired Nyquist entry of each
+// direction zeroed, k^2 not), p = -(D.f)/k^2 and u = (f - D p)/(eta k^2), i.e.
+// u = (f - d (d.f)/k^2)/(eta k^2) for the real d = D/i; the k = 0 mode is
+// dropped and the 1/(Nx Ny Nz) of the c2r transform folded in.  FP64.
+template <class T>
+__global__ void k_tp_solve(const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out,
+                           int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, double eta)
+{
+  typedef typename C2<T>::t V;
+  const int Nkx = Nx / 2 + 1;
+  const size_t nm = (size_t)Nz * Ny * Nkx;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int ix = (int)(m % Nkx);
+  const size_t r = m / Nkx;
+  const int iy = (int)(r % Ny), iz = (int)(r / Ny);
+  double ur[3] = {0, 0, 0}, ui[3] = {0, 0, 0};
+  if (m != 0) {
+    const double kx = kvec(ix, Nx, Lx), ky = kvec(iy, Ny, Ly), kz = kvec(iz, Nz, Lz);
+    const double ksq = kx * kx + ky * ky + kz * kz;
+    const double d[3] = {(Nx % 2 == 0 && ix == Nx / 2) ? 0.0 : kx, (Ny % 2 == 0 && iy == Ny / 2) ? 0.0 : ky,
+                         (Nz % 2 == 0 && iz == Nz / 2) ? 0.0 : kz};
+    double fr[3], fi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const V v = in[c * nm + m];
+      fr[c] = (double)v.x;
+      fi[c] = (double)v.y;
+    }
+    const double sr = d[0] * fr[0] + d[1] * fr[1] + d[2] * fr[2];
+    const double si = d[0] * fi[0] + d[1] * fi[1] + d[2] * fi[2];
+    const double iks = 1.0 / ksq;
+    const double val = iks / eta / ((double)Nx * (double)Ny * (double)Nz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ur[c] = val * (fr[c] - d[c] * sr * iks);
+      ui[c] = val * (fi[c] - d[c] * si * iks);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    V v;
+    v.x = (T)ur[c];
+    v.y = (T)ui[c];
+    out[c * nm + m] = v;
+  }
+}
+
+template <class T>
+void launch_tp_solve(const void* in, void* out, int Nx, int Ny, int Nz, double Lx, double Ly, double Lz,
+                     double eta, hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  const size_t nm = (size_t)Nz * Ny * (Nx / 2 + 1);
+  hipLaunchKernelGGL((k_tp_solve<T>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, (const V*)in,
+                     (V*)out, Nx, Ny, Nz, Lx, Ly, Lz, eta);
+}
+template void launch_tp_solve<float>(const void*, void*, int, int, int, double, double, double, double,
+                                     hipStream_t);
+template void launch_tp_solve<double>(const void*, void*, int, int, int, double, double, double, double,
+                                      hipStream_t);
 
 void launch_reset_err(int* e, hipStream_t s)
 {

@@ -37,7 +37,8 @@ import numpy as np
 
 from . import _native as N
 
-_MODES = {"nowall": N.DPS_MODE_NOWALL, "bottom": N.DPS_MODE_BOTTOM, "slit": N.DPS_MODE_SLIT}
+_MODES = {"nowall": N.DPS_MODE_NOWALL, "bottom": N.DPS_MODE_BOTTOM, "slit": N.DPS_MODE_SLIT,
+          "periodic": N.DPS_MODE_PERIODIC}
 _SOLVERS = {"auto": N.DPS_SOLVER_AUTO, "lane": N.DPS_SOLVER_LANE}
 
 
@@ -47,7 +48,7 @@ def getPrecision():
 
 
 class StokesParameters:
-    """Parameters of a doubly periodic Stokes solve (uammd.StokesParameters).
+    """Parameters of a doubly (or triply) periodic Stokes solve (uammd.StokesParameters).
 
     ``beta``/``beta_d`` are the ES exponents beta*w; ``<= 0`` selects 1.785*w.
     ``mode`` is 'nowall', 'bottom' or 'slit'.  ``solver`` picks the spectral
@@ -79,7 +80,8 @@ class StokesParameters:
 
     def to_c(self):
         if self.mode not in _MODES:
-            raise ValueError("mode must be one of 'nowall', 'bottom', 'slit' (got %r)" % (self.mode,))
+            raise ValueError("mode must be one of 'nowall', 'bottom', 'slit', 'periodic' (got %r)"
+                             % (self.mode,))
         if self.precision not in ("double", "mixed"):
             raise ValueError("precision must be 'double' or 'mixed'")
         if self.solver not in _SOLVERS:

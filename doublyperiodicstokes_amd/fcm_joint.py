@@ -15,8 +15,9 @@ Grid and kernels come from the same auto-configuration as the reference CPU
 backend (``gridconfig``), and the kernel support is the CPU's physical
 ``alpha = R_h / (2 c(beta))`` (``FCM.py:296-310``).  The z domain is the
 configured one, ``[a, a + Lz]`` with ``a = zmin - msep_down``, as the CPU path
-uses (``FCM.py:191-232``).  Triply periodic ``'TP'`` is not on this path and
-raises ``NotImplementedError``.  The reference's import-time guard on
+uses (``FCM.py:191-232``).  Triply periodic ``'TP'`` uses a uniform periodic z
+grid with ``hz = hx`` and ``Lz`` rounded up to whole cells
+(``GridAndKernelConfig.py:393-417``).  The reference's import-time guard on
 ``FCM_CPUCONFIG_LAUNCHED`` belongs to its CPU thread pinning and is not
 needed here.
 """
@@ -25,11 +26,11 @@ import numpy as np
 from .dpstokes import DPStokes, StokesParameters
 from .gridconfig import configure_grid_and_kernels_xy, configure_grid_and_kernels_z
 
-_DOM = {"DP": "nowall", "DPBW": "bottom", "DPSC": "slit"}
+_DOM = {"DP": "nowall", "DPBW": "bottom", "DPSC": "slit", "TP": "periodic"}
 
 
 class FCMJoint:
-    """Common interface for the doubly periodic FCM solvers."""
+    """Common interface for the doubly and triply periodic FCM solvers."""
 
     def __init__(self, device="hip", precision="double"):
         if device not in ("hip", "gpu"):
@@ -45,8 +46,6 @@ class FCMJoint:
                    verbose=False):
         """Configure grid and kernels and create the solver (re-callable)."""
         self.Clean()
-        if domType == "TP":
-            raise NotImplementedError("triply periodic domains are not on this path")
         if domType not in _DOM:
             raise ValueError("Domain type is not supported")
         self.has_torque = bool(has_torque)

@@ -37,10 +37,11 @@ extern "C" {
 #define DPS_ERR_FFT (-5)        /* rocFFT error */
 #define DPS_ERR_ALLOC (-6)      /* device allocation failed */
 
-/* geometry; mode strings of uammd_wrapper.cu:55-66 ("periodic" is not on this path) */
+/* geometry; mode strings of uammd_wrapper.cu:55-66 */
 #define DPS_MODE_NOWALL 1
 #define DPS_MODE_BOTTOM 2
 #define DPS_MODE_SLIT 3
+#define DPS_MODE_PERIODIC 4     /* triply periodic: uniform z, nz = Lz / hz points (Solvers.py:197-255) */
 
 /* arithmetic */
 #define DPS_PREC_DOUBLE 0       /* everything in FP64 */

@@ -51,7 +51,7 @@ def test_stokes_parameters_validation():
     assert (c.nx, c.ny, c.nz, c.mode, c.solver) == (32, 32, 17, 3, 0)
     assert "mode: slit" in str(p)
     with pytest.raises(ValueError):
-        StokesParameters(mode="periodic").to_c()          # triply periodic is not on this path
+        StokesParameters(mode="none").to_c()              # uammd's unknown-mode value
     with pytest.raises(ValueError):
         StokesParameters(mode="slit", precision="half").to_c()
     with pytest.raises(ValueError):
@@ -66,8 +66,6 @@ def test_joint_interface_argument_errors():
     with pytest.raises(ValueError):
         FCMJoint("cpu")
     s = FCMJoint("hip")
-    with pytest.raises(NotImplementedError):
-        s.Initialize(4, 1.0, 1.0, "TP", False, 20.0, 20.0, 0.0, 10.0)
     with pytest.raises(ValueError):
         s.Initialize(4, 1.0, 1.0, "XYZ", False, 20.0, 20.0, 0.0, 10.0)
     with pytest.raises(RuntimeError):
@@ -90,3 +88,14 @@ def test_no_device_fails_loudly():
                          mode="bottom", precision="double")
     with pytest.raises(RuntimeError):
         s.initialize(p, 4)
+
+
+def test_triply_periodic_configuration():
+    """'TP' (uammd mode 'periodic'): uniform z with hz = hx and Lz rounded up
+    to whole cells (GridAndKernelConfig.py:393-417)."""
+    from doublyperiodicstokes_amd import _native as N
+    from doublyperiodicstokes_amd.gridconfig import configure_grid_and_kernels_z
+    p = StokesParameters(mode="periodic", Lx=16.0, Ly=16.0, zmin=0.0, zmax=10.0, w=6, nx=32, ny=32, nz=20).to_c()
+    assert p.mode == N.DPS_MODE_PERIODIC == 4
+    Lz, hz, Nz, a = configure_grid_and_kernels_z(0.0, 10.3, 0.5, np.array([6]), np.array([6]), "TP")
+    assert hz == 0.5 and Nz == 21 and Lz == 10.5 and a == 0.0
