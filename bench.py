@@ -27,6 +27,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # kernel that runs each stage (the "bvp" stage is the fused spectral kernel)
 STAGE_KERNEL = {"spread": "k_spread", "fft_fwd": "rocfft", "z_fwd": "k_spectral", "bvp": "k_spectral",
                 "z_inv": "k_spectral", "fft_inv": "rocfft", "interp": "k_interp"}
+# unfused spectral chain (one wavenumber per lane)
+STAGE_KERNEL_LANE = dict(STAGE_KERNEL, z_fwd="k_zfwd", bvp="k_bvp", z_inv="k_zinv")
 
 WORKLOADS = {
     # name: (geometry, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque)
@@ -46,29 +48,34 @@ def make_inputs(wl, seed=0):
     return np.stack([x, y, z], 1).ravel(), F, T
 
 
-def stage_bytes(wl, s):
-    """Algorithmic HBM bytes of each stage (s = bytes per grid real)."""
+def stage_bytes(wl, s, reg=False):
+    """Algorithmic HBM bytes of each stage (s = bytes per grid real).  The
+    regular torque kernel doubles the grids and transforms, adds a curl to the
+    z-forward stage (read 2 spectra, write 1) and a z-forward, a curl and a
+    second inverse to the z-inverse stage."""
     geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
     ng = 3 * nx * ny * nz
     nm = 3 * nz * ny * (nx // 2 + 1)
     npart = N * (6 if torque else 3)
+    g = 2 if reg else 1
+    spec = nm * 2 * s
     return {
-        "spread": ng * s + npart * s,
-        "fft_fwd": ng * s + nm * 2 * s,
-        "z_fwd": 2 * nm * 2 * s,
-        "bvp": 2 * nm * 2 * s,
-        "z_inv": 2 * nm * 2 * s,
-        "fft_inv": nm * 2 * s + ng * s,
-        "interp": ng * s + npart * s,
+        "spread": g * ng * s + npart * s,
+        "fft_fwd": g * (ng * s + spec),
+        "z_fwd": g * 2 * spec + (3 * spec if reg else 0),
+        "bvp": 2 * spec,
+        "z_inv": 2 * spec + (3 * 2 * spec + 2 * spec if reg else 0),
+        "fft_inv": g * (spec + ng * s),
+        "interp": g * ng * s + npart * s,
     }
 
 
-def gpu_solver(wl, precision):
+def gpu_solver(wl, precision, reg=False):
     from doublyperiodicstokes_amd import DPStokes, StokesParameters
     geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
     par = StokesParameters(viscosity=1 / (6 * np.pi), Lx=Lx, Ly=Ly, zmin=0.0, zmax=zmax,
                            w=6, w_d=6 if torque else 0, beta=1.785 * 6, beta_d=1.785 * 6,
-                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision)
+                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision, useRegKernel=reg)
     s = DPStokes()
     s.initialize(par, N)
     return s
@@ -112,7 +119,7 @@ def copy_probe(torch, mb):
     del src, dst
 
 
-def cpu_baseline(wl, pos, F, T, budget_s):
+def cpu_baseline(wl, pos, F, T, budget_s, reg=False):
     """FP64 oracle Mdot timed on the host: as many full calls as fit in the budget (at least one)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import dp_oracle as o
@@ -120,7 +127,7 @@ def cpu_baseline(wl, pos, F, T, budget_s):
     np.seterr(all="ignore")
     ref = o.OracleDPStokes(geom, Lx, Ly, 0.0, zmax, nx, ny, nz, 1 / (6 * np.pi), 6, 1.785 * 6,
                            has_torque=torque, w_d=6 if torque else None,
-                           betaw_d=1.785 * 6 if torque else None)
+                           betaw_d=1.785 * 6 if torque else None, reg_kernel=reg)
     ref.set_positions(pos)
     n = 0
     t0 = time.perf_counter()
@@ -142,6 +149,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU-baseline Mdot work (0 disables)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--reg-kernel", action="store_true",
+                    help="torques through the regular kernel with spectral curls (useRegKernel)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy of this many MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
@@ -157,7 +166,8 @@ def main():
     Fd = torch.from_numpy(F.astype(np.float32)).to(dev)
     Td = torch.from_numpy(T.astype(np.float32)).to(dev) if torque else None
 
-    mixed = gpu_solver(wl, "mixed")
+    reg = args.reg_kernel and torque
+    mixed = gpu_solver(wl, "mixed", reg)
     mixed.setPositions(pos)
     ms = time_steps(torch, lambda: mixed.mdot_device(Fd, Td), args.steps, args.warmup)
     U32, W32 = mixed.mdot_device(Fd, Td)
@@ -173,13 +183,16 @@ def main():
         acc = t if acc is None else {k: acc[k] + t[k] for k in acc}
     mixed.set_profiling(False)
     stage_us = {k: v / nprof for k, v in acc.items() if k != "locate"}
+    kernels = STAGE_KERNEL
     if stage_us["z_fwd"] == 0 and stage_us["z_inv"] == 0:
         # fused spectral kernel: the z transforms run inside the "bvp" launch
         del stage_us["z_fwd"], stage_us["z_inv"]
-    sb = stage_bytes(wl, 4)
+    else:
+        kernels = STAGE_KERNEL_LANE
+    sb = stage_bytes(wl, 4, reg)
     stage_gbs = {k: sb[k] / (stage_us[k] * 1e-6) / 1e9 for k in stage_us}
     dom = max(stage_us, key=lambda k: stage_us[k])
-    traffic, traffic_src = measured_traffic(args.workload, STAGE_KERNEL[dom])
+    traffic, traffic_src = (None, None) if reg else measured_traffic(args.workload, kernels[dom])
 
     out = {
         "metric": "Mdot solves/sec, N=16k slit, mixed vs FP64 at rel-L2<=1e-5; spread HBM GB/s",
@@ -194,20 +207,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 grids + f64 BVP",
         "data": "synthetic",
-        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s"
-                   % (args.workload, geom, N, Lx, Ly, zmax, nx, ny, nz, "F+T" if torque else "F"),
+        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s%s"
+                   % (args.workload, geom, N, Lx, Ly, zmax, nx, ny, nz, "F+T" if torque else "F",
+                      " regular torque kernel" if reg else ""),
                    "precision": "mixed"},
         "stage_us": stage_us,
         "stage_GBps": stage_gbs,
         "spread_GBps": stage_gbs["spread"],
-        "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "stage": dom, "achieved": stage_gbs[dom],
+        "roofline": {"bound": "hbm", "kernel": kernels[dom], "stage": dom, "achieved": stage_gbs[dom],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stage_gbs[dom] / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": sb[dom],
                      "launch_us": stage_us[dom]},
     }
     del mixed
     if not args.no_fp64:
-        f64 = gpu_solver(wl, "double")
+        f64 = gpu_solver(wl, "double", reg)
         f64.setPositions(pos)
         Fd64 = torch.from_numpy(F).to(dev)
         Td64 = torch.from_numpy(T).to(dev) if torque else None
@@ -224,7 +238,7 @@ def main():
     if args.copy_probe_mb > 0:
         copy_probe(torch, args.copy_probe_mb)
     if args.cpu_budget > 0:
-        rate, n, el, thr = cpu_baseline(wl, pos, F, T, args.cpu_budget)
+        rate, n, el, thr = cpu_baseline(wl, pos, F, T, args.cpu_budget, reg)
         out["cpu_baseline"] = {"value": rate, "unit": "Mdot/s", "cores": thr, "kind": "port",
                                "sample": "%d full-size FP64 oracle Mdot call(s) on the same inputs (%.1f s)"
                                % (n, el)}

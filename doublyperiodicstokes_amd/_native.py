@@ -42,7 +42,8 @@ class DpsParams(ctypes.Structure):
                 ("beta", ctypes.c_double), ("beta_d", ctypes.c_double),
                 ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("mode", ctypes.c_int), ("precision", ctypes.c_int),
-                ("solver", ctypes.c_int)]
+                ("solver", ctypes.c_int),
+                ("reg_kernel", ctypes.c_int)]
 
 
 class DpsError(RuntimeError):

@@ -57,6 +57,7 @@ struct dps_handle {
   int Nx = 0, Ny = 0, Nz = 0, Nkx = 0, Nm = 0;
   KernelDesc km{}, kd{};
   bool has_dip = false;
+  bool reg = false;   // regular torque kernel with spectral curls
   ZExt zm, zd;
   std::vector<double> zpts, zwts;
   ChebTables cheb;
@@ -75,6 +76,8 @@ struct dps_handle {
   int *d_key = nullptr, *d_rank = nullptr, *d_counts = nullptr, *d_start = nullptr, *d_order = nullptr;
   void* d_grid = nullptr;
   void *d_specA = nullptr, *d_specB = nullptr;
+  // regular torque kernel: torque / half-vorticity grid and two more spectra
+  void *d_gridT = nullptr, *d_specT = nullptr, *d_specC = nullptr;
   double *d_zext_m = nullptr, *d_zw_m = nullptr, *d_zext_d = nullptr, *d_zw_d = nullptr;
   double *d_fA_m = nullptr, *d_fB_m = nullptr, *d_fA_d = nullptr, *d_fB_d = nullptr;
   int *d_kt_m = nullptr, *d_kt_d = nullptr;
@@ -203,8 +206,10 @@ int build(dps_handle* h)
   int rc = setup_kernel(h, P.w, P.alpha, P.beta, false, &h->km);
   if (rc) return rc;
   h->has_dip = P.w_d > 0;
+  if (P.reg_kernel != 0 && P.reg_kernel != 1) return fail(DPS_ERR_ARG, "reg_kernel must be 0 or 1");
+  h->reg = h->has_dip && P.reg_kernel == 1;
   if (h->has_dip) {
-    rc = setup_kernel(h, P.w_d, P.alpha_d, P.beta_d, true, &h->kd);
+    rc = setup_kernel(h, P.w_d, P.alpha_d, P.beta_d, !h->reg, &h->kd);
     if (rc) return rc;
   }
   const int wmax = std::max(std::max(h->km.wx, h->km.wy), h->has_dip ? std::max(h->kd.wx, h->kd.wy) : 0);
@@ -231,7 +236,9 @@ int build(dps_handle* h)
   }
   h->cheb = build_cheb_tables(h->Nz, h->H);
   h->ct = build_cheb_transform(h->Nz);
-  h->fused = (P.solver == DPS_SOLVER_AUTO) && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
+  // the regular torque kernel's curls act on Chebyshev coefficients between
+  // the unfused z-transform and BVP kernels
+  h->fused = (P.solver == DPS_SOLVER_AUTO) && !h->reg && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
   if (h->fused) {
     // up to 64 KiB per workgroup keeps several workgroups per CU; beyond
     // that, what the device allows, else the unfused chain
@@ -276,7 +283,9 @@ int build_tiles(dps_handle* h, int wmax)
   // [wx][wy][dx][dy][z spread][dz spread][z interp][dz interp], z parts by
   // interior row from sup.z
   h->rs_m = h->km.wx + h->km.wy + 2 * h->km.wzmax;
-  h->rs_d = h->has_dip ? 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax : 0;
+  h->rs_d = !h->has_dip ? 0
+            : h->reg ? h->kd.wx + h->kd.wy + 2 * h->kd.wzmax
+                     : 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
   {
     // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
     SpreadArgs S{};
@@ -324,6 +333,11 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc((unsigned char**)&h->d_grid, ng * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specA, nspec * 2 * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specB, nspec * 2 * ts))) return rc;
+  if (h->reg) {
+    if ((rc = h->alloc((unsigned char**)&h->d_gridT, ng * ts))) return rc;
+    if ((rc = h->alloc((unsigned char**)&h->d_specT, nspec * 2 * ts))) return rc;
+    if ((rc = h->alloc((unsigned char**)&h->d_specC, nspec * 2 * ts))) return rc;
+  }
   // triply periodic: the r2c spectrum [c][kz][ky][kx] has the same size, and
   // nothing else is needed
   if (h->geometry == DPS_MODE_PERIODIC) return DPS_OK;
@@ -540,6 +554,7 @@ StencilArgs stencil_args(dps_handle* h)
   A.zrange = h->d_zrange;
   A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
   A.hz = h->hz;
+  A.dip_regular = h->reg;
   return A;
 }
 
@@ -556,6 +571,21 @@ SpreadArgs spread_args(dps_handle* h)
   A.rs_max = std::max(h->rs_m, h->rs_d);
   A.grid = h->d_grid;
   A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
+  A.zr_dip = 0;
+  return A;
+}
+
+// the torque kernel presented as the only (kind-0) kernel: a plain ES pass
+// over the torque records (regular torque kernel)
+SpreadArgs spread_args_torque(dps_handle* h)
+{
+  SpreadArgs A = spread_args(h);
+  A.km = h->kd;
+  A.sup_m = h->d_sup_d;
+  A.rec_m = h->d_rec_d;
+  A.rs_m = h->rs_d;
+  A.grid = h->d_gridT;
+  A.zr_dip = 1;
   return A;
 }
 
@@ -633,16 +663,28 @@ void mark(dps_handle* h, int i, hipStream_t s)
   if (h->prof) (void)hipEventRecord(h->ev[i], s);
 }
 
-int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_t s)
+void spread_pass(const SpreadArgs& A, const void* F, const void* Tq, bool mx, int dtype, hipStream_t s)
 {
-  const SpreadArgs A = spread_args(h);
-  const void* Tq = (T && h->has_dip) ? T : nullptr;
-  if (mixed(h)) {
+  if (mx) {
     if (dtype == DPS_F32) launch_spread<float, float>(A, F, Tq, s);
     else launch_spread<float, double>(A, F, Tq, s);
   } else {
     if (dtype == DPS_F32) launch_spread<double, float>(A, F, Tq, s);
     else launch_spread<double, double>(A, F, Tq, s);
+  }
+}
+
+// forces (and the torque curl of the derivative kernels) into d_grid; with
+// the regular torque kernel the torques go, as a plain vector field, into
+// d_gridT
+int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_t s)
+{
+  const void* Tq = (T && h->has_dip) ? T : nullptr;
+  if (h->reg) {
+    spread_pass(spread_args(h), F, nullptr, mixed(h), dtype, s);
+    if (Tq) spread_pass(spread_args_torque(h), Tq, nullptr, mixed(h), dtype, s);
+  } else {
+    spread_pass(spread_args(h), F, Tq, mixed(h), dtype, s);
   }
   HIPCHK(hipGetLastError());
   return DPS_OK;
@@ -698,16 +740,112 @@ int do_solve(dps_handle* h, hipStream_t s, int ev0)
   return DPS_OK;
 }
 
+// Spectral stage of the regular torque kernel (FCM.py:372-396, 427-439;
+// Solvers.py:137-179).  d_grid holds the spread forces and d_gridT the spread
+// torques; on return d_grid holds the velocity and d_gridT half the vorticity.
+//   doubly periodic: z-forward both spectra, add half the Chebyshev curl of
+//   the torques to the forcing, solve, z-inverse (wall corrections included),
+//   z-forward that velocity again -- the curl acts on the coefficients of the
+//   corrected velocity, as the reference's does -- take half its curl
+//   (rescaled by Nx Ny, which the inverse tables divide out) and z-inverse it
+//   without corrections;
+//   triply periodic: the same with Fourier curls around the Fourier solve.
+// Events as do_solve: 2 forward FFTs, 3 z forward + torque curl, 4 BVP,
+// 5 z inverse + vorticity, 6 inverse FFTs (triply periodic: 2, 4, 6).
+int do_solve_reg(dps_handle* h, hipStream_t s)
+{
+  int rc;
+  const bool mx = mixed(h);
+  if ((rc = run_fft(h, true, h->d_grid, h->d_specA, s))) return rc;
+  if ((rc = run_fft(h, true, h->d_gridT, h->d_specT, s))) return rc;
+  mark(h, 2, s);
+  if (h->geometry == DPS_MODE_PERIODIC) {
+    const double Lx = h->P.Lx, Ly = h->P.Ly, Lz = h->Lz, eta = h->P.viscosity;
+    if (mx) {
+      launch_tp_curl<float>(h->d_specT, h->d_specA, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, 1, s);
+      launch_tp_solve<float>(h->d_specA, h->d_specB, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, eta, s);
+      launch_tp_curl<float>(h->d_specB, h->d_specT, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, 0, s);
+    } else {
+      launch_tp_curl<double>(h->d_specT, h->d_specA, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, 1, s);
+      launch_tp_solve<double>(h->d_specA, h->d_specB, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, eta, s);
+      launch_tp_curl<double>(h->d_specB, h->d_specT, h->Nx, h->Ny, h->Nz, Lx, Ly, Lz, 0, s);
+    }
+    HIPCHK(hipGetLastError());
+    mark(h, 4, s);
+    if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+    if ((rc = run_fft(h, false, h->d_specT, h->d_gridT, s))) return rc;
+    mark(h, 6, s);
+    return DPS_OK;
+  }
+  const ZArgs Z = z_args(h);
+  ZArgs Z0 = Z;
+  Z0.nbasis = 0;   // the vorticity's inverse carries no wall correction
+  const double nxy = (double)h->Nx * (double)h->Ny;
+  const BvpArgs B = bvp_args(h);
+  if (mx) {
+    launch_zfwd<float>(Z, h->d_specA, h->d_specB, s);
+    launch_zfwd<float>(Z, h->d_specT, h->d_specC, s);
+    launch_curl_cheb<float>(h->d_specC, h->d_specB, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, 1.0, 1, s);
+  } else {
+    launch_zfwd<double>(Z, h->d_specA, h->d_specB, s);
+    launch_zfwd<double>(Z, h->d_specT, h->d_specC, s);
+    launch_curl_cheb<double>(h->d_specC, h->d_specB, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, 1.0, 1, s);
+  }
+  HIPCHK(hipGetLastError());
+  mark(h, 3, s);
+  if (mx) launch_bvp<float>(B, h->d_specB, h->d_specA, s);
+  else launch_bvp<double>(B, h->d_specB, h->d_specA, s);
+  HIPCHK(hipGetLastError());
+  mark(h, 4, s);
+  if (mx) {
+    launch_zinv<float>(Z, h->d_specA, h->d_specB, s);
+    launch_zfwd<float>(Z, h->d_specB, h->d_specC, s);
+    launch_curl_cheb<float>(h->d_specC, h->d_specT, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+    launch_zinv<float>(Z0, h->d_specT, h->d_specC, s);
+  } else {
+    launch_zinv<double>(Z, h->d_specA, h->d_specB, s);
+    launch_zfwd<double>(Z, h->d_specB, h->d_specC, s);
+    launch_curl_cheb<double>(h->d_specC, h->d_specT, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+    launch_zinv<double>(Z0, h->d_specT, h->d_specC, s);
+  }
+  HIPCHK(hipGetLastError());
+  mark(h, 5, s);
+  if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+  if ((rc = run_fft(h, false, h->d_specC, h->d_gridT, s))) return rc;
+  mark(h, 6, s);
+  return DPS_OK;
+}
+
+void interp_pass(const InterpArgs& A, void* vel, void* ang, bool mx, int dtype, hipStream_t s)
+{
+  if (mx) {
+    if (dtype == DPS_F32) launch_interp<float, float>(A, vel, ang, s);
+    else launch_interp<float, double>(A, vel, ang, s);
+  } else {
+    if (dtype == DPS_F32) launch_interp<double, float>(A, vel, ang, s);
+    else launch_interp<double, double>(A, vel, ang, s);
+  }
+}
+
+// velocities from d_grid; angular velocities from the derivative kernels on
+// d_grid, or (regular torque kernel) the torque kernel on the half-vorticity
+// grid d_gridT
 int do_interp(dps_handle* h, void* vel, void* ang, int dtype, hipStream_t s)
 {
-  const InterpArgs A = interp_args(h);
   void* a = (ang && h->has_dip) ? ang : nullptr;
-  if (mixed(h)) {
-    if (dtype == DPS_F32) launch_interp<float, float>(A, vel, a, s);
-    else launch_interp<float, double>(A, vel, a, s);
+  if (h->reg) {
+    interp_pass(interp_args(h), vel, nullptr, mixed(h), dtype, s);
+    if (a) {
+      InterpArgs A = interp_args(h);
+      A.km = h->kd;
+      A.sup_m = h->d_sup_d;
+      A.rec_m = h->d_rec_d;
+      A.rs_m = h->rs_d;
+      A.grid = h->d_gridT;
+      interp_pass(A, a, nullptr, mixed(h), dtype, s);
+    }
   } else {
-    if (dtype == DPS_F32) launch_interp<double, float>(A, vel, a, s);
-    else launch_interp<double, double>(A, vel, a, s);
+    interp_pass(interp_args(h), vel, a, mixed(h), dtype, s);
   }
   HIPCHK(hipGetLastError());
   return DPS_OK;
@@ -836,7 +974,7 @@ int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* veloc
   mark(h, 0, s);
   if ((rc = do_spread(h, forces, torques, dtype, s))) return rc;
   mark(h, 1, s);
-  if ((rc = do_solve(h, s, 1))) return rc;
+  if ((rc = (h->reg && torques) ? do_solve_reg(h, s) : do_solve(h, s, 1))) return rc;
   if ((rc = do_interp(h, velocities, torques ? angular : nullptr, dtype, s))) return rc;
   mark(h, 7, s);
   h->have_timings = h->prof;
@@ -848,6 +986,8 @@ int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  if (torques && h->reg)
+    return fail(DPS_ERR_ARG, "with the regular torque kernel the torque curl is spectral: no real-space torque grid");
   hipStream_t s = (hipStream_t)stream;
   if ((rc = do_spread(h, forces, torques, dtype, s))) return rc;
   if (mixed(h)) launch_grid_to_ref<float>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
@@ -880,6 +1020,8 @@ int dps_stage_interp(dps_handle* h, const double* grid_in, void* velocities, voi
 {
   int rc = check_handle(h, true);
   if (rc) return rc;
+  if (angular && h->reg)
+    return fail(DPS_ERR_ARG, "with the regular torque kernel angular velocities need the spectral vorticity");
   hipStream_t s = (hipStream_t)stream;
   if (mixed(h)) launch_ref_to_grid<float>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
   else launch_ref_to_grid<double>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);

@@ -53,6 +53,7 @@ struct StencilArgs {
   int4* zrange;          // per particle: (klo, khi) monopole, (klo, khi) dipole
   int periodic_z;        // triply periodic: uniform z nodes k hz, wrapped like x and y
   double hz;
+  int dip_regular;       // the torque kernel is a plain ES kernel: records in the monopole layout
 };
 
 struct SpreadArgs {
@@ -66,6 +67,8 @@ struct SpreadArgs {
   const void *rec_m, *rec_d;
   int rs_m, rs_d, rs_max;
   void* grid;   // [3][Nz][Ny][Nx] grid precision
+  int zr_dip;   // kind-0 entries cull by the dipole z range (a pass that spreads torques
+                // with the torque kernel's records given as km/sup_m/rec_m)
 };
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
 // words staged per thread and spreading entry for records of rs_max words (0: too long)
@@ -83,6 +86,18 @@ struct InterpArgs {
   const void* grid;
   int periodic_z;   // rows of a support wrap modulo Nz
 };
+
+// Half the curl of a Fourier-Chebyshev field [c][k][Nm] (x, y derivatives i kx,
+// i ky with the unpaired Nyquist entries zeroed; z derivative by the Chebyshev
+// coefficient recurrence on [0, 2H]), times `scale`; added to `out` or stored.
+template <class T>
+void launch_curl_cheb(const void* in, void* out, int Nx, int Ny, int Nz, int Nm, const double* kxs,
+                      const double* kys, double H, double scale, int add, hipStream_t s);
+// Half the Fourier curl of an r2c spectrum [c][kz][ky][kx] (unpaired Nyquist
+// entries of each direction zeroed); added to `out` or stored.
+template <class T>
+void launch_tp_curl(const void* in, void* out, int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int add,
+                    hipStream_t s);
 
 // Fourier solve of the triply periodic Stokes problem on the r2c spectrum
 // [c][kz][ky][kx] (kx in [0, Nx/2]), scaled by 1/(Nx Ny Nz) for the c2r

@@ -90,6 +90,9 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
   A.pos[3 * p + 2] = z;
   const double hxhy = A.hx * A.hy;
   for (int kind = 0; kind < (A.has_dip ? 2 : 1); ++kind) {
+    // deriv: the torque kernel's derivative records; a regular torque kernel
+    // (spectral curl) keeps the monopole layout
+    const bool deriv = kind && !A.dip_regular;
     const KernelDesc& K = kind ? A.kd : A.km;
     const double* zext = kind ? A.zext_d : A.zext_m;
     const double* zwe = kind ? A.zw_d : A.zw_m;
@@ -107,14 +110,14 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
     sup.x = s.bx; sup.y = s.by; sup.z = s.e0; sup.w = s.nz;
     T* rec = (T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
     double wx[DPS_MAX_W], dx[DPS_MAX_W], wy[DPS_MAX_W], dy[DPS_MAX_W];
-    stencil_weights_xy(x, A.hx, s.bx, K.wx, K, K.inv_norm3, wx, kind ? dx : nullptr);
-    stencil_weights_xy(y, A.hy, s.by, K.wy, K, 1.0, wy, kind ? dy : nullptr);
+    stencil_weights_xy(x, A.hx, s.bx, K.wx, K, K.inv_norm3, wx, deriv ? dx : nullptr);
+    stencil_weights_xy(y, A.hy, s.by, K.wy, K, 1.0, wy, deriv ? dy : nullptr);
     int o = 0;
     for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)wx[i];
     o += K.wx;
     for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)wy[j];
     o += K.wy;
-    if (kind) {
+    if (deriv) {
       for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)dx[i];
       o += K.wx;
       for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
@@ -131,7 +134,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
         const double v = clamp_off((double)(bz + q) * A.hz - z, K.thresh);
         const double ph = es_phi(v, K.betaw, a2);
         const double qw = hxhy * A.hz;
-        if (kind == 0) {
+        if (!deriv) {
           rec[o + q] = (T)ph;
           rec[o + W + q] = (T)(ph * qw);
         } else {
@@ -164,24 +167,24 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
     const int W = K.wzmax;
     const int* kt = kind ? A.kt_d : A.kt_m;
     const double* fa = kind ? A.fA_d : A.fA_m;
-    const double* fb = kind ? A.fB_d : nullptr;
+    const double* fb = deriv ? A.fB_d : nullptr;
     int klo = 1 << 30, khi = -1;
     for (int q = 0; q < s.nz; ++q) {
       const int e = s.e0 + q;
-      if (fa[e] == 0.0 && (kind ? fb[e] : 0.0) == 0.0) continue;
+      if (fa[e] == 0.0 && (deriv ? fb[e] : 0.0) == 0.0) continue;
       klo = min(klo, kt[e]);
       khi = max(khi, kt[e]);
     }
-    for (int i = 0; i < (kind ? 4 : 2) * W; ++i) rec[o + i] = (T)0;
+    for (int i = 0; i < (deriv ? 4 : 2) * W; ++i) rec[o + i] = (T)0;
     for (int q = 0; q < s.nz; ++q) {
       const int e = s.e0 + q;
-      const double sa = fa[e], sb = kind ? fb[e] : 0.0;
+      const double sa = fa[e], sb = deriv ? fb[e] : 0.0;
       if (sa == 0.0 && sb == 0.0) continue;
       const double v = clamp_off(zext[e] - z, K.thresh);
       const double ph = es_phi(v, K.betaw, a2);
       const double qw = hxhy * zwe[e];
       const int r = kt[e] - klo;
-      if (kind == 0) {
+      if (!deriv) {
         rec[o + r] += (T)(ph * sa);                 // spread
         rec[o + W + r] += (T)(ph * qw * sa);        // interpolation
       } else {
@@ -371,7 +374,8 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       const int p = A.order[tstart[j] + (ci - tpre[j])];
       const int4 zr = A.zrange[p];
       for (int kind = 0; kind < nkind; ++kind) {
-        const int klo = kind ? zr.z : zr.x, khi = kind ? zr.w : zr.y;
+        const bool zd = kind || A.zr_dip;
+        const int klo = zd ? zr.z : zr.x, khi = zd ? zr.w : zr.y;
         // rows klo..khi, and their periodic images when z is periodic
         bool hz = false;
         for (int sh = A.periodic_z ? -1 : 0; sh <= (A.periodic_z ? 1 : 0); ++sh)
@@ -1041,6 +1045,132 @@ template void launch_tp_solve<float>(const void*, void*, int, int, int, double, 
                                      hipStream_t);
 template void launch_tp_solve<double>(const void*, void*, int, int, int, double, double, double, double,
                                       hipStream_t);
+
+// Half curl of a Fourier-Chebyshev field (the regular-kernel torque path,
+// FCM.py:372-396 for the torque grid, Solvers.py:159-179 for the velocity).
+// One thread per (kx, ky) mode walks the levels from the top coefficient down,
+// carrying the z-derivative recurrence of the x and y components
+// (Chebyshev.py:68-94): D_k = D_{k+2} + (2/H)(k+1) f_{k+1}, D_0 halved.
+// Loads and stores are coalesced across modes.  FP64 arithmetic.
+template <class T>
+__global__ void k_curl_cheb(const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out,
+                            int Nx, int Ny, int Nz, int Nm, const double* __restrict__ kxs,
+                            const double* __restrict__ kys, double two_over_H, double scale, int add)
+{
+  typedef typename C2<T>::t V;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= Nm) return;
+  const int Nkx = Nx / 2 + 1;
+  const int iy = m / Nkx, ix = m - iy * Nkx;
+  const double dx = (Nx % 2 == 0 && ix == Nx / 2) ? 0.0 : kxs[ix];
+  const double dy = (Ny % 2 == 0 && iy == Ny / 2) ? 0.0 : kys[iy];
+  const size_t cs = (size_t)Nz * Nm;
+  const double h = 0.5 * scale;
+  // recurrence state of the x and y components: D_{k+1}, D_{k+2}, f_{k+1}
+  double x1r = 0, x1i = 0, x2r = 0, x2i = 0, fxr = 0, fxi = 0;
+  double y1r = 0, y1i = 0, y2r = 0, y2i = 0, fyr = 0, fyi = 0;
+  for (int k = Nz - 1; k >= 0; --k) {
+    const size_t o = (size_t)k * Nm + m;
+    const V a0 = in[o], a1 = in[o + cs], a2 = in[o + 2 * cs];
+    const double c = two_over_H * (double)(k + 1);
+    double zxr = x2r + c * fxr, zxi = x2i + c * fxi;   // d/dz of component x at level k
+    double zyr = y2r + c * fyr, zyi = y2i + c * fyi;
+    x2r = x1r; x2i = x1i; x1r = zxr; x1i = zxi;
+    y2r = y1r; y2i = y1i; y1r = zyr; y1i = zyi;
+    fxr = (double)a0.x; fxi = (double)a0.y;
+    fyr = (double)a1.x; fyi = (double)a1.y;
+    if (k == 0) { zxr *= 0.5; zxi *= 0.5; zyr *= 0.5; zyi *= 0.5; }
+    const double azr = (double)a2.x, azi = (double)a2.y;
+    // (i dy az - dz ay, dz ax - i dx az, i dx ay - i dy ax) / 2
+    double w[6];
+    w[0] = h * (-dy * azi - zyr);
+    w[1] = h * (dy * azr - zyi);
+    w[2] = h * (zxr + dx * azi);
+    w[3] = h * (zxi - dx * azr);
+    w[4] = h * (-dx * fyi + dy * fxi);
+    w[5] = h * (dx * fyr - dy * fxr);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      V v;
+      if (add) {
+        const V b = out[o + cc * cs];
+        v.x = (T)((double)b.x + w[2 * cc]);
+        v.y = (T)((double)b.y + w[2 * cc + 1]);
+      } else {
+        v.x = (T)w[2 * cc];
+        v.y = (T)w[2 * cc + 1];
+      }
+      out[o + cc * cs] = v;
+    }
+  }
+}
+
+template <class T>
+void launch_curl_cheb(const void* in, void* out, int Nx, int Ny, int Nz, int Nm, const double* kxs,
+                      const double* kys, double H, double scale, int add, hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  hipLaunchKernelGGL((k_curl_cheb<T>), dim3((Nm + 255) / 256), dim3(256), 0, s, (const V*)in, (V*)out, Nx, Ny, Nz,
+                     Nm, kxs, kys, 2.0 / H, scale, add);
+}
+template void launch_curl_cheb<float>(const void*, void*, int, int, int, int, const double*, const double*, double,
+                                      double, int, hipStream_t);
+template void launch_curl_cheb<double>(const void*, void*, int, int, int, int, const double*, const double*,
+                                       double, double, int, hipStream_t);
+
+// Half the Fourier curl (FCM.py:376-383, Solvers.py:137-157): w = (i d) x a / 2
+// with d the wavevector, the unpaired Nyquist entry of each direction zeroed.
+template <class T>
+__global__ void k_tp_curl(const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out,
+                          int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int add)
+{
+  typedef typename C2<T>::t V;
+  const int Nkx = Nx / 2 + 1;
+  const size_t nm = (size_t)Nz * Ny * Nkx;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int ix = (int)(m % Nkx);
+  const size_t r = m / Nkx;
+  const int iy = (int)(r % Ny), iz = (int)(r / Ny);
+  const double d0 = (Nx % 2 == 0 && ix == Nx / 2) ? 0.0 : kvec(ix, Nx, Lx);
+  const double d1 = (Ny % 2 == 0 && iy == Ny / 2) ? 0.0 : kvec(iy, Ny, Ly);
+  const double d2 = (Nz % 2 == 0 && iz == Nz / 2) ? 0.0 : kvec(iz, Nz, Lz);
+  double ar[3], ai[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const V v = in[c * nm + m];
+    ar[c] = (double)v.x;
+    ai[c] = (double)v.y;
+  }
+  // (d x a) componentwise, then times i / 2: (re, im) -> (-im, re) / 2
+  const double cr[3] = {d1 * ar[2] - d2 * ar[1], d2 * ar[0] - d0 * ar[2], d0 * ar[1] - d1 * ar[0]};
+  const double ci[3] = {d1 * ai[2] - d2 * ai[1], d2 * ai[0] - d0 * ai[2], d0 * ai[1] - d1 * ai[0]};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double wr = -0.5 * ci[c], wi = 0.5 * cr[c];
+    if (add) {
+      const V b = out[c * nm + m];
+      wr += (double)b.x;
+      wi += (double)b.y;
+    }
+    V v;
+    v.x = (T)wr;
+    v.y = (T)wi;
+    out[c * nm + m] = v;
+  }
+}
+
+template <class T>
+void launch_tp_curl(const void* in, void* out, int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int add,
+                    hipStream_t s)
+{
+  typedef typename C2<T>::t V;
+  const size_t nm = (size_t)Nz * Ny * (Nx / 2 + 1);
+  hipLaunchKernelGGL((k_tp_curl<T>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, (const V*)in, (V*)out,
+                     Nx, Ny, Nz, Lx, Ly, Lz, add);
+}
+template void launch_tp_curl<float>(const void*, void*, int, int, int, double, double, double, int, hipStream_t);
+template void launch_tp_curl<double>(const void*, void*, int, int, int, double, double, double, int, hipStream_t);
 
 void launch_reset_err(int* e, hipStream_t s)
 {

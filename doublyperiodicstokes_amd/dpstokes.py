@@ -53,12 +53,15 @@ class StokesParameters:
     ``beta``/``beta_d`` are the ES exponents beta*w; ``<= 0`` selects 1.785*w.
     ``mode`` is 'nowall', 'bottom' or 'slit'.  ``solver`` picks the spectral
     solve: 'auto' (fused lane-group kernel when nz <= 512) or 'lane' (one
-    wavenumber per lane, separate z-transform kernels).
+    wavenumber per lane, separate z-transform kernels).  ``useRegKernel``
+    selects the CPU reference's regular-kernel torque path (FCM.py:149,
+    372-396, 427-439): torques spread with the plain ES kernel of the dipole
+    width and curls taken spectrally (it runs the 'lane' kernels).
     """
 
     def __init__(self, viscosity=1.0, Lx=0.0, Ly=0.0, zmin=0.0, zmax=0.0, w=1, w_d=1,
                  alpha=-1.0, alpha_d=-1.0, beta=-1.0, beta_d=-1.0, nx=-1, ny=-1, nz=-1,
-                 mode="none", precision=None, solver="auto"):
+                 mode="none", precision=None, solver="auto", useRegKernel=False):
         self.viscosity = viscosity
         self.Lx, self.Ly = Lx, Ly
         self.zmin, self.zmax = zmin, zmax
@@ -70,6 +73,7 @@ class StokesParameters:
         self.precision = precision if precision is not None else (
             "mixed" if getPrecision() == "single" else "double")
         self.solver = solver
+        self.useRegKernel = bool(useRegKernel)
 
     def __str__(self):
         return ("viscosity: %g\nLx: %g\nLy: %g\nzmin: %g\nzmax: %g\nw: %g\nw_d: %g\nalpha: %g\n"
@@ -95,6 +99,7 @@ class StokesParameters:
         p.beta, p.beta_d = float(self.beta), float(self.beta_d)
         p.nx, p.ny, p.nz = int(self.nx), int(self.ny), int(self.nz)
         p.solver = _SOLVERS[self.solver]
+        p.reg_kernel = 1 if self.useRegKernel else 0
         p.mode = _MODES[self.mode]
         p.precision = N.DPS_PREC_MIXED if self.precision == "mixed" else N.DPS_PREC_DOUBLE
         return p

@@ -43,8 +43,11 @@ class FCMJoint:
 
     def Initialize(self, numberParticles, hydrodynamicRadius, viscosity, domType, has_torque,
                    xmax, ymax, zmin, zmax, xmin=0, ymin=0, kernType=0, optInd=0, ref=False,
-                   verbose=False):
-        """Configure grid and kernels and create the solver (re-callable)."""
+                   verbose=False, useRegKernel=False):
+        """Configure grid and kernels and create the solver (re-callable).
+        ``useRegKernel`` is the keyword python_interface/dpstokes_common.py:19-31
+        passes: torques through the plain ES kernel and spectral curls
+        (FCM.py:149, 372-396, 427-439) instead of the derivative kernels."""
         self.Clean()
         if domType not in _DOM:
             raise ValueError("Domain type is not supported")
@@ -67,7 +70,8 @@ class FCMJoint:
             beta_d=float(beta_d[0] * w_d[0]) if self.has_torque else -1.0,
             alpha=float(self.radP[0] / (2 * cbeta[0])),
             alpha_d=float(self.radP[0] / (2 * cbeta_d[0])) if self.has_torque else -1.0,
-            nx=nx, ny=ny, nz=nz, mode=_DOM[domType], precision=self.grid_precision)
+            nx=nx, ny=ny, nz=nz, mode=_DOM[domType], precision=self.grid_precision,
+            useRegKernel=useRegKernel)
         self.par = par
         self.solver = DPStokes()
         self.solver.initialize(par, numberParticles)

@@ -64,6 +64,15 @@ extern "C" {
  *   w_d/beta_d/alpha_d describe the derivative (dipole) kernel used for torques.
  *   solver : DPS_SOLVER_* (no counterpart in the reference; both give the same
  *            result to rounding).
+ *   reg_kernel : 0 = torques through the derivative kernels (default);
+ *            1 = the CPU reference's useRegKernel option (FCM.py:149, 257,
+ *            372-396, 427-439): torques are spread with the plain ES kernel
+ *            of the dipole width (w_d, beta_d, alpha_d; support alpha_d), half
+ *            their spectral curl joins the forcing, and angular velocities
+ *            are the same kernel applied to half the spectral curl of the
+ *            velocity (Fourier in x, y; Chebyshev coefficient derivative in
+ *            z, Fourier in z when periodic).  Uses the one-wavenumber-per-lane
+ *            spectral kernels whatever `solver` says.
  */
 typedef struct dps_params {
   double viscosity;
@@ -76,6 +85,7 @@ typedef struct dps_params {
   int mode;
   int precision;
   int solver;
+  int reg_kernel;
 } dps_params;
 
 typedef struct dps_handle dps_handle;
@@ -116,7 +126,9 @@ int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* veloc
 /* Stage entry points (for stage-level parity tests and the adjoint):
    spread forces (+torque curl) to the real grid, returned in the reference's
    interior layout (Ny, Nx, Nz, 3) FP64; solve grid -> velocity grid (same
-   layout, FP64); interpolate a velocity grid (same layout) to particles. */
+   layout, FP64); interpolate a velocity grid (same layout) to particles.
+   With reg_kernel = 1 the torque curl lives in spectral space only: torques
+   to dps_stage_spread and angular to dps_stage_interp are DPS_ERR_ARG. */
 int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int dtype,
                      double* grid_out, void* stream);
 int dps_stage_solve(dps_handle* h, const double* grid_in, double* grid_out, void* stream);

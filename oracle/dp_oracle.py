@@ -135,6 +135,25 @@ def cheb_coeff_diff_py(fhat, Nz, H):
     return Df
 
 
+def curl_cheb(C, Dx, Dy, H):
+    """Half the curl of a Fourier-Chebyshev field C (modes, Nz, 3): x and y
+    derivatives by Dx, Dy per mode, the z derivative by the coefficient
+    recurrence (FCM.py:372-396 for the torque grid, Solvers.py:159-179 for the
+    velocity; Chebyshev.py:68-94)."""
+    Nz = C.shape[1]
+
+    def dz(a):
+        return cheb_coeff_diff_py(a.T, Nz, H).T
+
+    Dx = Dx[:, None]
+    Dy = Dy[:, None]
+    out = np.empty_like(C)
+    out[..., 0] = 0.5 * (Dy * C[..., 2] - dz(C[..., 1]))
+    out[..., 1] = 0.5 * (dz(C[..., 0]) - Dx * C[..., 2])
+    out[..., 2] = 0.5 * (Dx * C[..., 1] - Dy * C[..., 0])
+    return out
+
+
 def first_integral_matrix(N, H):
     """First Chebyshev integral matrix (N x N+2), H-scaled; Chebyshev.py:119-137."""
     F = np.zeros((N, N + 2))
@@ -525,7 +544,8 @@ class OracleDPStokes:
     """
 
     def __init__(self, geometry, Lx, Ly, zmin, zmax, nx, ny, nz, eta,
-                 w, betaw, alpha=None, w_d=None, betaw_d=None, alpha_d=None, has_torque=False):
+                 w, betaw, alpha=None, w_d=None, betaw_d=None, alpha_d=None, has_torque=False,
+                 reg_kernel=False):
         if geometry not in ('nowall', 'bottom', 'slit'):
             raise ValueError('geometry must be nowall, bottom or slit')
         self.geometry = geometry
@@ -543,7 +563,10 @@ class OracleDPStokes:
         if has_torque:
             if alpha_d is None or alpha_d < 0:
                 alpha_d = w_d * self.hx / 2
-            self.dip = KernelSpec(w_d, betaw_d, alpha_d, dipole=True)
+            # useRegKernel (FCM.py:313-319): the torque kernel is the plain ES
+            # kernel of the dipole width, support alpha_d
+            self.dip = KernelSpec(w_d, betaw_d, alpha_d, dipole=not reg_kernel)
+        self.reg_kernel = bool(reg_kernel) and has_torque
         # BC tables (FCM.py:215-248): [end*3 + comp], ends x-,x+,y-,y+,z-(top),z+(bottom)
         self.BCs = BC_NONE * np.ones(18, dtype=np.int32)
         if geometry == 'bottom':
@@ -632,11 +655,17 @@ class OracleDPStokes:
 
     def spread_forces(self, forces, torques=None):
         """Spread + fold of forces, plus the torque curl (FCM.py:354-371,
-        SpreadInterpWrapper.cpp:66-107).  Returns the interior grid."""
+        SpreadInterpWrapper.cpp:66-107).  Returns the interior grid.  With the
+        regular torque kernel the torques are spread as a vector field into
+        self.tgrid and their curl is taken spectrally by solve()."""
         self._ext_f = self._ext_buffer(self._loc)
         F = self._spread(0, self._loc, self.mono, forces, self.BCs, self._ext_f)
+        self.tgrid = None
         if self.has_torque and torques is not None and np.size(torques) > 0:
             self._ext_t = self._ext_buffer(self._tloc)
+            if self.reg_kernel:
+                self.tgrid = self._spread(0, self._tloc, self.dip, torques, self.BCs, self._ext_t)
+                return F
             dxT = self._spread(1, self._tloc, self.dip, torques, self.BCs, self._ext_t)
             F[..., 1] -= 0.5 * dxT[..., 2]
             F[..., 2] += 0.5 * dxT[..., 1]
@@ -648,12 +677,18 @@ class OracleDPStokes:
             F[..., 1] += 0.5 * dzT[..., 0]
         return F
 
-    def solve(self, F):
+    def solve(self, F, tgrid=None):
         """Forward transform, per-mode solve, corrections, k=0, back transform.
-        FCM.py:398-407 with Solvers.py:99-136."""
+        FCM.py:398-407 with Solvers.py:99-136.  With a torque grid (regular
+        torque kernel, FCM.py:372-396 and 427-433) half its spectral curl is
+        added to the forcing, and half the curl of the velocity coefficients
+        is transformed back into self.utG."""
         Ny, Nx, Nz = self.Ny, self.Nx, self.Nz
         Fh = fc_forward(F)
         Cf = Fh.reshape(Ny * Nx, Nz, 3)
+        if tgrid is not None:
+            Th = fc_forward(tgrid).reshape(Ny * Nx, Nz, 3)
+            Cf = Cf + curl_cheb(Th, self.ops.Dx, self.ops.Dy, self.H)
         U, P = dp_solve_no_wall(self.ops, Cf, self.eta)
         if self.geometry == 'bottom':
             bottom_wall_correction(self.ops, U, P, self.zpts, self.eta)
@@ -663,13 +698,21 @@ class OracleDPStokes:
             cf0 = Cf[0].real + 1j * Cf[0].real   # Solvers.py:564-566
             k0_wall_velocity(self.ops, cf0, U, self.eta, 'slit')
         self.U_hat = U
+        self.utG = None
+        if tgrid is not None:
+            W = curl_cheb(U, self.ops.Dx, self.ops.Dy, self.H)
+            self.utG = fc_backward(W.reshape(Ny, Nx, Nz, 3))
         return fc_backward(U.reshape(Ny, Nx, Nz, 3))
 
-    def interpolate(self, uG, torques_present):
+    def interpolate(self, uG, torques_present, utG=None):
         v = self._interp(0, self._loc, self.mono, uG, self.BCs, self._ext_f)
         if not torques_present:
             return v, None
         ext = self._ext_t
+        if self.reg_kernel:
+            # angular velocity: the regular kernel applied to half the curl
+            # grid (FCM.py:437-439)
+            return v, self._interp(0, self._tloc, self.dip, utG, self.BCs, ext)
         dxv = self._interp(1, self._tloc, self.dip, uG, self.BCs, ext)
         dyv = self._interp(2, self._tloc, self.dip, uG, self.BCs, ext)
         dzv = self._interp(3, self._tloc, self.dip, uG, self.BCst_z, ext)
@@ -688,9 +731,9 @@ class OracleDPStokes:
         if tp and np.size(torques) != 3 * self.N:
             raise ValueError('Dimension mismatch in particle force array')
         F = self.spread_forces(forces, torques if tp else None)
-        uG = self.solve(F)
+        uG = self.solve(F, self.tgrid)
         self.uG = uG
-        return self.interpolate(uG, tp)
+        return self.interpolate(uG, tp, self.utG)
 
 
 def num_threads():

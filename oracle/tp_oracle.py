@@ -61,7 +61,7 @@ class OracleTPStokes:
     """FP64 CPU Mdot, triply periodic box [0,Lx) x [0,Ly) x [zmin, zmin+Lz)."""
 
     def __init__(self, Lx, Ly, zmin, zmax, nx, ny, nz, eta, w, betaw, alpha=None,
-                 w_d=None, betaw_d=None, alpha_d=None, has_torque=False):
+                 w_d=None, betaw_d=None, alpha_d=None, has_torque=False, reg_kernel=False):
         self.Lx, self.Ly, self.zmin = float(Lx), float(Ly), float(zmin)
         self.Lz = float(zmax) - float(zmin)
         self.Nx, self.Ny, self.Nz = int(nx), int(ny), int(nz)
@@ -74,7 +74,10 @@ class OracleTPStokes:
         if has_torque:
             if alpha_d is None or alpha_d < 0:
                 alpha_d = w_d * self.hx / 2
-            self.dip = KernelSpec(w_d, betaw_d, alpha_d, dipole=True)
+            # useRegKernel (FCM.py:313-319): plain ES kernel of the dipole
+            # width, support alpha_d
+            self.dip = KernelSpec(w_d, betaw_d, alpha_d, dipole=not reg_kernel)
+        self.reg_kernel = bool(reg_kernel) and has_torque
 
     def set_positions(self, positions):
         xs = np.array(positions, dtype=np.float64).reshape(-1, 3).copy()
@@ -122,9 +125,16 @@ class OracleTPStokes:
         return np.stack([np.sum(U[g, d] * W, axis=(1, 2, 3)) for d in range(3)], 1)
 
     def spread_forces(self, forces, torques=None):
+        """Force grid plus the torque curl; with the regular torque kernel the
+        torques are spread as a vector field into self.tgrid instead and their
+        curl is taken in solve()."""
         F = self._spread(0, self._loc, self.mono, np.reshape(forces, (-1, 3)))
+        self.tgrid = None
         if self.has_torque and torques is not None and np.size(torques) > 0:
             T = np.reshape(torques, (-1, 3))
+            if self.reg_kernel:
+                self.tgrid = self._spread(0, self._tloc, self.dip, T)
+                return F
             dxT = self._spread(1, self._tloc, self.dip, T)
             dyT = self._spread(2, self._tloc, self.dip, T)
             dzT = self._spread(3, self._tloc, self.dip, T)
@@ -149,9 +159,23 @@ class OracleTPStokes:
             Dz[:, :, self.Nz // 2] = 0
         return Ksq, Dx, Dy, Dz
 
-    def solve(self, F):
+    @staticmethod
+    def curl_hat(C, Dx, Dy, Dz):
+        """Half the Fourier curl of C (..., 3) (FCM.py:376-395, Solvers.py:137-157)."""
+        out = np.empty_like(C)
+        out[..., 0] = 0.5 * (Dy * C[..., 2] - Dz * C[..., 1])
+        out[..., 1] = 0.5 * (Dz * C[..., 0] - Dx * C[..., 2])
+        out[..., 2] = 0.5 * (Dx * C[..., 1] - Dy * C[..., 0])
+        return out
+
+    def solve(self, F, tgrid=None):
+        """Fourier solve; with a torque grid (regular torque kernel) half its
+        curl is added to the forcing and half the curl of the velocity is
+        transformed back into self.utG."""
         Fh = np.fft.fftn(F, axes=(0, 1, 2))
         Ksq, Dx, Dy, Dz = self._operators()
+        if tgrid is not None:
+            Fh = Fh + self.curl_hat(np.fft.fftn(tgrid, axes=(0, 1, 2)), Dx, Dy, Dz)
         Ksq = Ksq.copy()
         Ksq[0, 0, 0] = 1.0
         f, g, h = Fh[..., 0], Fh[..., 1], Fh[..., 2]
@@ -159,17 +183,22 @@ class OracleTPStokes:
         val = 1.0 / (self.eta * Ksq)
         U = np.stack([val * (f - Dx * p), val * (g - Dy * p), val * (h - Dz * p)], -1)
         U[0, 0, 0, :] = 0
+        self.utG = None
+        if tgrid is not None:
+            self.utG = np.real(np.fft.ifftn(self.curl_hat(U, Dx, Dy, Dz), axes=(0, 1, 2)))
         return np.real(np.fft.ifftn(U, axes=(0, 1, 2)))
 
     def mdot(self, forces, torques=None):
         forces = np.asarray(forces, dtype=np.float64)
         tp = self.has_torque and torques is not None and np.size(torques) > 0
         F = self.spread_forces(forces, torques if tp else None)
-        uG = self.solve(F)
+        uG = self.solve(F, self.tgrid)
         self.uG = uG
         v = self._interp(0, self._loc, self.mono, uG).ravel()
         if not tp:
             return v, None
+        if self.reg_kernel:
+            return v, self._interp(0, self._tloc, self.dip, self.utG).ravel()
         dxv = self._interp(1, self._tloc, self.dip, uG)
         dyv = self._interp(2, self._tloc, self.dip, uG)
         dzv = self._interp(3, self._tloc, self.dip, uG)

@@ -71,7 +71,10 @@ def test_argument_errors_are_reported_without_a_device(lib):
     assert lib.dps_create(ctypes.byref(p), 4, ctypes.byref(h)) == N.DPS_ERR_ARG
     p.precision, p.solver = N.DPS_PREC_MIXED, 9
     assert lib.dps_create(ctypes.byref(p), 4, ctypes.byref(h)) == N.DPS_ERR_ARG
-    p.solver, p.zmax = N.DPS_SOLVER_AUTO, -1.0
+    p.solver, p.reg_kernel = N.DPS_SOLVER_AUTO, 2
+    assert lib.dps_create(ctypes.byref(p), 4, ctypes.byref(h)) == N.DPS_ERR_ARG
+    assert b"reg_kernel" in lib.dps_last_error()
+    p.reg_kernel, p.zmax = 0, -1.0
     assert lib.dps_create(ctypes.byref(p), 4, ctypes.byref(h)) == N.DPS_ERR_ARG
     assert lib.dps_destroy(None) == N.DPS_OK
     assert lib.dps_get_precision(None) == b"double"

@@ -48,7 +48,9 @@ def test_stokes_parameters_validation():
     p = StokesParameters(viscosity=1.0, Lx=8, Ly=8, zmin=0, zmax=4, w=6, w_d=6, nx=32, ny=32, nz=17,
                          mode="slit", precision="double")
     c = p.to_c()
-    assert (c.nx, c.ny, c.nz, c.mode, c.solver) == (32, 32, 17, 3, 0)
+    assert (c.nx, c.ny, c.nz, c.mode, c.solver, c.reg_kernel) == (32, 32, 17, 3, 0, 0)
+    p.useRegKernel = True
+    assert p.to_c().reg_kernel == 1
     assert "mode: slit" in str(p)
     with pytest.raises(ValueError):
         StokesParameters(mode="none").to_c()              # uammd's unknown-mode value
