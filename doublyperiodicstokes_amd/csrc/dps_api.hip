@@ -99,6 +99,8 @@ struct dps_handle {
   double *d_fib = nullptr, *d_fab = nullptr, *d_fG = nullptr;
 
   rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
+  bool fft_y = false;        // rocFFT runs the x pass only; k_fft_y the y pass
+  void* d_twy = nullptr;     // y-pass twiddles exp(-2 pi i k / Ny)
   rocfft_execution_info info = nullptr;
   void* fft_work = nullptr;
   size_t fft_work_size = 0;
@@ -486,7 +488,21 @@ int make_plans(dps_handle* h)
   const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
   const bool tp = h->geometry == DPS_MODE_PERIODIC;
   size_t lengths[3] = {(size_t)h->Nx, (size_t)h->Ny, (size_t)h->Nz};
-  const size_t dim = tp ? 3 : 2, batch = tp ? 3 : (size_t)3 * h->Nz;
+  // doubly periodic with a power-of-two Ny: rocFFT does the unit-stride x pass
+  // (batched over component, z and y) and k_fft_y the strided y pass
+  h->fft_y = !tp && fft_y_supported(h->Ny);
+  const size_t dim = tp ? 3 : (h->fft_y ? 1 : 2);
+  const size_t batch = tp ? 3 : (size_t)3 * h->Nz * (h->fft_y ? h->Ny : 1);
+  if (h->fft_y) {
+    std::vector<double> tw(2 * (size_t)h->Ny);
+    for (int k = 0; k < h->Ny; ++k) {
+      tw[2 * k] = std::cos(2 * M_PI * k / h->Ny);
+      tw[2 * k + 1] = -std::sin(2 * M_PI * k / h->Ny);
+    }
+    if (h->prec == DPS_PREC_MIXED) rc = h->upload((float**)&h->d_twy, cast_vec<float>(tw));
+    else rc = h->upload((double**)&h->d_twy, tw);
+    if (rc) return rc;
+  }
   FFTCHK(rocfft_plan_create(&h->plan_fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
                             fp, dim, lengths, batch, nullptr));
   FFTCHK(rocfft_plan_create(&h->plan_inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
@@ -504,12 +520,25 @@ int make_plans(dps_handle* h)
 
 bool mixed(const dps_handle* h) { return h->prec == DPS_PREC_MIXED; }
 
+// planar transform grid -> spectrum (forward) or spectrum -> grid (inverse);
+// with the split plan the inverse's y pass runs in place on `in`
 int run_fft(dps_handle* h, bool forward, void* in, void* out, hipStream_t s)
 {
+  const int ns = 3 * h->Nz;
+  if (h->fft_y && !forward) {
+    if (mixed(h)) launch_fft_y<float>(in, ns, h->Nkx, h->Ny, h->d_twy, 1, s);
+    else launch_fft_y<double>(in, ns, h->Nkx, h->Ny, h->d_twy, 1, s);
+    HIPCHK(hipGetLastError());
+  }
   FFTCHK(rocfft_execution_info_set_stream(h->info, s));
   void* ib[1] = {in};
   void* ob[1] = {out};
   FFTCHK(rocfft_execute(forward ? h->plan_fwd : h->plan_inv, ib, ob, h->info));
+  if (h->fft_y && forward) {
+    if (mixed(h)) launch_fft_y<float>(out, ns, h->Nkx, h->Ny, h->d_twy, -1, s);
+    else launch_fft_y<double>(out, ns, h->Nkx, h->Ny, h->d_twy, -1, s);
+    HIPCHK(hipGetLastError());
+  }
   return DPS_OK;
 }
 

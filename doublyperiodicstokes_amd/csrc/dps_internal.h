@@ -99,6 +99,13 @@ template <class T>
 void launch_tp_curl(const void* in, void* out, int Nx, int Ny, int Nz, double Lx, double Ly, double Lz, int add,
                     hipStream_t s);
 
+// y pass of the planar transforms: in-place c2c of length Ny along ky of the
+// spectrum [nslices][Ny][Nkx], dir -1 forward / +1 inverse (unnormalised);
+// tw holds exp(-2 pi i k / Ny), k < Ny, in grid precision.
+bool fft_y_supported(int Ny);
+template <class T>
+void launch_fft_y(void* data, int nslices, int Nkx, int Ny, const void* tw, int dir, hipStream_t s);
+
 // Fourier solve of the triply periodic Stokes problem on the r2c spectrum
 // [c][kz][ky][kx] (kx in [0, Nx/2]), scaled by 1/(Nx Ny Nz) for the c2r
 template <class T>

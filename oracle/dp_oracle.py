@@ -597,6 +597,13 @@ class OracleDPStokes:
     def _locate(self, xs, ks):
         L = lib()
         zext, zw, Nzeff, up, dn = self._zext(ks.thresh)
+        if up != dn:
+            # the fold and ghost copy split the extension into equal halves
+            # (BoundaryConditions.h:22-361); a node at exactly the support
+            # distance from one end but not, after rounding, from the other
+            # breaks that
+            raise ValueError('z extension is asymmetric (%d above, %d below): the support '
+                             'edge falls on a Chebyshev node' % (up, dn))
         N = xs.shape[0] // 3
         wx = int(round(2 * ks.thresh / self.hx))
         wy = int(round(2 * ks.thresh / self.hy))

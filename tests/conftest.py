@@ -20,7 +20,7 @@ def corecheck():
     out = os.path.join(ROOT, "tests", "native", "_build", "libcorecheck.so")
     src = os.path.join(ROOT, "tests", "native", "core_check.cpp")
     hdr = [os.path.join(ROOT, "doublyperiodicstokes_amd", "csrc", h)
-           for h in ("dps_core.h", "dps_seg.h", "dps_tables.h")]
+           for h in ("dps_core.h", "dps_fft.h", "dps_seg.h", "dps_tables.h")]
     if not os.path.exists(out) or any(os.path.getmtime(f) > os.path.getmtime(out) for f in [src] + hdr):
         os.makedirs(os.path.dirname(out), exist_ok=True)
         subprocess.check_call(["g++", "-O2", "-std=c++20", "-shared", "-fPIC", "-pthread",

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <vector>
 #include "dps_core.h"
+#include "dps_fft.h"
 #include "dps_seg.h"
 #include "dps_tables.h"
 
@@ -200,6 +201,23 @@ void cc_cheb_transform(int N, double* Tf /*N*N*/, double* Ti /*N*N*/)
       Tf[k * N + j] = sgn * f;
       Ti[j * N + k] = sgn * i;
     }
+}
+
+// register DFT of the y-pass kernel: R in {2, 4, 8, 16, 32}, dir -1 / +1
+int cc_fft_reg(int R, int dir, double* re, double* im)
+{
+#define CC_FFT(RR)                                                    \
+  if (R == RR) {                                                      \
+    double a[RR], b[RR];                                              \
+    for (int i = 0; i < RR; ++i) { a[i] = re[i]; b[i] = im[i]; }      \
+    if (dir < 0) fft_reg<RR, -1>(a, b);                               \
+    else fft_reg<RR, 1>(a, b);                                        \
+    for (int i = 0; i < RR; ++i) { re[i] = a[i]; im[i] = b[i]; }      \
+    return 0;                                                         \
+  }
+  CC_FFT(2) CC_FFT(4) CC_FFT(8) CC_FFT(16) CC_FFT(32)
+#undef CC_FFT
+  return -1;
 }
 
 double cc_es_norm(double alpha, double betaw) { return es_norm(alpha, betaw); }

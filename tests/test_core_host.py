@@ -253,3 +253,18 @@ def test_stencils_match_oracle_locate(corecheck):
         np.testing.assert_allclose(Wz[p * wzmax:p * wzmax + nz], phi(zu), rtol=1e-14)
         np.testing.assert_allclose(Wzi[p * wzmax:p * wzmax + nz], phi(zu) * loc["pt_wts"][p * zs:p * zs + nz],
                                    rtol=1e-14)
+
+
+@pytest.mark.parametrize("R", [2, 4, 8, 16, 32])
+def test_register_fft_matches_numpy(corecheck, R):
+    """The y-pass kernel's register DFT (dps_fft.h) against numpy's FFT, both
+    directions (inverse unnormalised)."""
+    import ctypes
+    rng = np.random.RandomState(R)
+    x = rng.normal(size=R) + 1j * rng.normal(size=R)
+    for d, want in ((-1, np.fft.fft(x)), (1, R * np.fft.ifft(x))):
+        re = np.ascontiguousarray(x.real)
+        im = np.ascontiguousarray(x.imag)
+        p = ctypes.POINTER(ctypes.c_double)
+        assert corecheck.cc_fft_reg(R, d, re.ctypes.data_as(p), im.ctypes.data_as(p)) == 0
+        assert np.abs(re + 1j * im - want).max() <= 1e-14 * np.abs(want).max()

@@ -1,0 +1,37 @@
+"""The planar transforms with the y pass in k_fft_y (every power-of-two Ny the
+kernel handles, odd and even Nx so that the flattened (slice, kx) column tiles
+straddle slices at different offsets) against the FP64 oracle, through Mdot
+with torques.  Bounds as everywhere: rel-L2 <= 1e-11 FP64, <= 1e-5 mixed."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import TOL64, TOLMIX, gpu_for, make_case, oracle_for, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+# name, geometry, Lx, Ly, Lz, nx, ny, nz, N, torque
+CASES = [
+    ("fy16", "nowall", 8.0, 8.0, 6.3, 16, 16, 13, 20, True),
+    ("fy64_oddx", "slit", 13.5, 32.0, 6.3, 27, 64, 13, 30, True),
+    ("fy128", "bottom", 12.0, 64.0, 6.0, 24, 128, 11, 30, True),
+    ("fy256_oddx", "slit", 7.5, 128.0, 5.0, 15, 256, 9, 30, True),
+    ("fy512", "nowall", 8.0, 256.0, 6.0, 16, 512, 11, 30, True),
+]
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fft_y_pass_mdot(case, precision):
+    say("case %s %s y-pass FFT mdot" % (case[0], precision))
+    pos, F, T = make_case(case, seed=7)
+    ref = oracle_for(case)
+    ref.set_positions(pos)
+    v_ref, w_ref = ref.mdot(F, T)
+    s = gpu_for(case, precision)
+    s.setPositions(pos)
+    v, w = s.Mdot(F, T)
+    tol = TOL64 if precision == "double" else TOLMIX
+    ev, ew = rel_l2(v, v_ref), rel_l2(w, w_ref)
+    say("  rel-L2 U = %.3e  W = %.3e" % (ev, ew))
+    assert ev <= tol and ew <= tol
+    s.clear()

@@ -353,3 +353,13 @@ def test_mobility_is_symmetric_positive_definite(geometry):
     asym = np.linalg.norm(M - M.T) / np.linalg.norm(M)
     assert asym < 1e-3, asym
     assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+
+
+def test_asymmetric_z_extension_is_refused():
+    """Lz = 6, Nz = 13, support 1.5: Chebyshev nodes lie exactly one support
+    width from each wall, rounding extends the grid on one side only, and the
+    reference's equal-halves fold would run past the grid; the oracle refuses
+    (as the GPU build does) instead."""
+    ref = o.OracleDPStokes('nowall', 8.0, 8.0, 0.0, 6.0, 16, 16, 13, 1.0, 6, 1.785 * 6)
+    with pytest.raises(ValueError, match="asymmetric"):
+        ref.set_positions(np.array([4.0, 4.0, 3.0]))
