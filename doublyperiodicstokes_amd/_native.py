@@ -28,7 +28,7 @@ DPS_F32 = 0
 DPS_F64 = 1
 
 EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions", "dps_mdot",
-           "dps_stage_spread",
+           "dps_mdot_adjoint", "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
            "dps_get_timings", "dps_get_precision", "dps_last_error")
 
@@ -71,6 +71,7 @@ def load():
     L.dps_set_positions.argtypes = [vp, vp, ctypes.c_int, vp]
     L.dps_update_positions.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
     L.dps_mdot.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
+    L.dps_mdot_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_stage_spread.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp]
     L.dps_stage_solve.argtypes = [vp, vp, vp, vp]
     L.dps_stage_interp.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]

@@ -78,6 +78,13 @@ struct dps_handle {
   void *d_specA = nullptr, *d_specB = nullptr;
   // regular torque kernel: torque / half-vorticity grid and two more spectra
   void *d_gridT = nullptr, *d_specT = nullptr, *d_specC = nullptr;
+  // adjoint (allocated on first use): second velocity grid, force-kernel
+  // derivative records
+  void* d_gridA = nullptr;
+  int4* d_sup_md = nullptr;
+  void* d_rec_md = nullptr;
+  int4* d_zrange_md = nullptr;
+  int rs_md = 0;
   double *d_zext_m = nullptr, *d_zw_m = nullptr, *d_zext_d = nullptr, *d_zw_d = nullptr;
   double *d_fA_m = nullptr, *d_fB_m = nullptr, *d_fA_d = nullptr, *d_fB_d = nullptr;
   int *d_kt_m = nullptr, *d_kt_d = nullptr;
@@ -544,7 +551,7 @@ int run_fft(dps_handle* h, bool forward, void* in, void* out, hipStream_t s)
 
 StencilArgs stencil_args(dps_handle* h)
 {
-  StencilArgs A;
+  StencilArgs A{};
   A.N = h->N;
   A.xmin = 0;
   A.ymin = 0;
@@ -584,12 +591,13 @@ StencilArgs stencil_args(dps_handle* h)
   A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
   A.hz = h->hz;
   A.dip_regular = h->reg;
+  A.first_kind = 0;
   return A;
 }
 
 SpreadArgs spread_args(dps_handle* h)
 {
-  SpreadArgs A;
+  SpreadArgs A{};
   A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
   A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty; A.RX = h->RX; A.RY = h->RY;
   A.start = h->d_start; A.order = h->d_order;
@@ -620,7 +628,7 @@ SpreadArgs spread_args_torque(dps_handle* h)
 
 InterpArgs interp_args(dps_handle* h)
 {
-  InterpArgs A;
+  InterpArgs A{};
   A.N = h->N; A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
   A.order = h->d_order;
   A.km = h->km; A.kd = h->kd;
@@ -635,7 +643,7 @@ InterpArgs interp_args(dps_handle* h)
 
 ZArgs z_args(dps_handle* h)
 {
-  ZArgs A;
+  ZArgs A{};
   A.N = h->Nz; A.NH = h->ct.NH; A.NE = h->ct.NE; A.NO = h->ct.NO;
   A.Nm = h->Nm; A.Nkx = h->Nkx;
   A.TfE = h->d_TfE; A.TfO = h->d_TfO; A.TiE = h->d_TiE; A.TiO = h->d_TiO;
@@ -649,7 +657,7 @@ ZArgs z_args(dps_handle* h)
 
 BvpArgs bvp_args(dps_handle* h)
 {
-  BvpArgs A;
+  BvpArgs A{};
   BvpConsts& B = A.B;
   B.Nz = h->Nz; B.geometry = h->geometry; B.H = h->H; B.Hs = h->Lz; B.eta = h->P.viscosity;
   B.sd0 = h->d_sd0; B.sm2 = h->d_sm2; B.sp2 = h->d_sp2;
@@ -672,7 +680,7 @@ BvpArgs bvp_args(dps_handle* h)
 
 SpecArgs spec_args(dps_handle* h)
 {
-  SpecArgs A;
+  SpecArgs A{};
   A.B = bvp_args(h).B;
   A.Nx = h->Nx; A.Ny = h->Ny; A.Nkx = h->Nkx; A.Nm = h->Nm;
   A.Lx = h->P.Lx; A.Ly = h->P.Ly;
@@ -880,6 +888,53 @@ int do_interp(dps_handle* h, void* vel, void* ang, int dtype, hipStream_t s)
   return DPS_OK;
 }
 
+// buffers of the position adjoint, allocated on its first call
+int ensure_adjoint(dps_handle* h)
+{
+  if (h->d_gridA) return DPS_OK;
+  int rc;
+  const size_t ng = (size_t)3 * h->Nz * h->Ny * h->Nx;
+  h->rs_md = 2 * h->km.wx + 2 * h->km.wy + 4 * h->km.wzmax;
+  if ((rc = h->alloc(&h->d_sup_md, h->N))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_rec_md, (size_t)h->N * h->rs_md * h->tsz))) return rc;
+  if ((rc = h->alloc(&h->d_zrange_md, h->N))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_gridA, ng * h->tsz))) return rc;
+  return DPS_OK;
+}
+
+// force-kernel derivative records (dipole layout, ghost factors of the force
+// kernel in x, y and z, zero derivative where an offset was clamped) from the
+// located positions
+KernelDesc force_deriv_kernel(const dps_handle* h)
+{
+  KernelDesc K = h->km;
+  K.pos_deriv = 1;
+  return K;
+}
+
+int locate_force_derivatives(dps_handle* h, hipStream_t s)
+{
+  StencilArgs A = stencil_args(h);
+  A.first_kind = 1;
+  A.has_dip = 1;
+  A.dip_regular = 0;
+  A.kd = force_deriv_kernel(h);
+  A.zmin = 0;   // d_pos is already shifted and wrapped
+  A.zext_d = h->d_zext_m;
+  A.zw_d = h->d_zw_m;
+  A.kt_d = h->d_kt_m;
+  A.fA_d = h->d_fA_m;
+  A.fB_d = h->d_fA_m;
+  A.sup_d = h->d_sup_md;
+  A.rec_d = h->d_rec_md;
+  A.rs_d = h->rs_md;
+  A.zrange = h->d_zrange_md;
+  if (mixed(h)) launch_stencil<double, float>(A, h->d_pos, s);
+  else launch_stencil<double, double>(A, h->d_pos, s);
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
 int check_handle(const dps_handle* h, bool need_pos)
 {
   if (!h) return fail(DPS_ERR_STATE, "DPStokes is not initialized. Call Initialize first");
@@ -1008,6 +1063,64 @@ int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* veloc
   mark(h, 7, s);
   h->have_timings = h->prof;
   return DPS_OK;
+}
+
+int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, const void* grad_velocities,
+                     const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
+                     int dtype, void* stream)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (h->N > 0 && !grad_velocities) return fail(DPS_ERR_ARG, "null grad_velocities");
+  if ((grad_angular || grad_torques || torques) && !h->has_dip)
+    return fail(DPS_ERR_ARG, "torque terms given but no dipole kernel (w_d = 0)");
+  if (grad_torques && (!grad_forces || !grad_angular))
+    return fail(DPS_ERR_ARG, "grad_torques needs grad_forces and grad_angular (zeros if W is unused)");
+  if (!grad_positions) {
+    // the solve is taken as self-adjoint: [dL/dF; dL/dT] = M [gU; gW]
+    if (!grad_forces) return DPS_OK;
+    return dps_mdot(h, grad_velocities, grad_angular, grad_forces, grad_torques, dtype, stream);
+  }
+  if (torques || grad_angular || grad_torques)
+    return fail(DPS_ERR_ARG, "position gradients are available for forces only (no torque terms)");
+  if (h->N > 0 && !forces) return fail(DPS_ERR_ARG, "null forces");
+  if ((rc = ensure_adjoint(h))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool p = h->prof;
+  h->prof = false;
+  rc = locate_force_derivatives(h, s);
+  // u = L S F into d_gridA, v = L S gU into d_grid
+  if (!rc) rc = do_spread(h, forces, nullptr, dtype, s);
+  if (!rc) rc = do_solve(h, s, 1);
+  std::swap(h->d_grid, h->d_gridA);
+  if (!rc) rc = do_spread(h, grad_velocities, nullptr, dtype, s);
+  if (!rc) rc = do_solve(h, s, 1);
+  if (!rc && grad_forces) rc = do_interp(h, grad_forces, nullptr, dtype, s);
+  if (!rc) {
+    const InterpArgs A = interp_args(h);
+    const KernelDesc K = force_deriv_kernel(h);
+    if (mixed(h)) {
+      if (dtype == DPS_F32)
+        launch_interp_grad<float, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                         grad_velocities, forces, grad_positions, s);
+      else
+        launch_interp_grad<float, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                          grad_velocities, forces, grad_positions, s);
+    } else {
+      if (dtype == DPS_F32)
+        launch_interp_grad<double, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                          grad_velocities, forces, grad_positions, s);
+      else
+        launch_interp_grad<double, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                           grad_velocities, forces, grad_positions, s);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPS_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  }
+  std::swap(h->d_grid, h->d_gridA);
+  h->prof = p;
+  return rc;
 }
 
 int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int dtype, double* grid_out,

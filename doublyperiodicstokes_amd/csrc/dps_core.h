@@ -65,6 +65,8 @@ struct KernelDesc {
   double thresh;     // cutoff (alpha for the monopole, < alpha for the dipole)
   double inv_norm3;  // 1 / norm^3
   int wx, wy;        // support points in x and y: round(2 thresh / hx), round(2 thresh / hy)
+  int pos_deriv;     // derivative weights as functions of the particle position: zero where
+                     // the offset was clamped (the weight is then the constant phi(thresh))
   int nzeff;         // points of the z-extended grid
   int ext_up;        // mirrored ghost nodes above the top
   int wzmax;         // bound on the z support size
@@ -138,7 +140,7 @@ DPS_HD void stencil_weights_xy(double pos, double h, int b, int w, const KernelD
     const double v = clamp_off((double)(b + i) * h - pos, k.thresh);
     const double p = es_phi(v, k.betaw, a2) * scale;
     if (wout) wout[i] = p;
-    if (dout) dout[i] = p * es_dfac(v, k.betaw, a2);
+    if (dout) dout[i] = (k.pos_deriv && fabs(v) >= k.thresh) ? 0.0 : p * es_dfac(v, k.betaw, a2);
   }
 }
 
@@ -153,7 +155,7 @@ DPS_HD void stencil_weights_z(double z, const Stencil& s, const KernelDesc& k, c
     const int e = s.e0 + q;
     const double v = clamp_off(zext[e] - z, k.thresh);
     const double p = es_phi(v, k.betaw, a2);
-    const double d = p * es_dfac(v, k.betaw, a2);
+    const double d = (k.pos_deriv && fabs(v) >= k.thresh) ? 0.0 : p * es_dfac(v, k.betaw, a2);
     const double qw = hxhy * zwext[e];
     if (wout) wout[q] = p;
     if (dout) dout[q] = d;

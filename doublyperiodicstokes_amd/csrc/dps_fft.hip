@@ -6,9 +6,9 @@
 // over (slice, kx) and a workgroup takes TW consecutive ones, so that every
 // load and store of the tile touches TW consecutive complex values of a row
 // (256 bytes) -- the tile may straddle two slices, which only splits a row
-// access in two.  The length N = R1 R2 transform of each column is done in
-// two register passes with one exchange through LDS (n = n1 + R1 n2,
-// k = k2 + R2 k1):
+// access in two.  Consecutive tiles run on the same XCD.  The length N = R1 R2
+// transform of each column is done in two register passes with one exchange
+// through LDS (n = n1 + R1 n2, k = k2 + R2 k1):
 //   Y[n1][k2] = W_N^{n1 k2} sum_n2 x[n1 + R1 n2] W_R2^{n2 k2}
 //   X[k2 + R2 k1] = sum_n1 Y[n1][k2] W_R1^{n1 k1}
 // in place, so the spectrum crosses HBM once each way.
@@ -37,7 +37,9 @@ __global__ void __launch_bounds__(256) k_fft_y(typename C2<T>::t* __restrict__ d
   constexpr int G1 = 256 / TW;   // column groups
   __shared__ V buf[N * TW];       // [n1][k2][col]
   const int col = threadIdx.x % TW, g = threadIdx.x / TW;
-  const int q = blockIdx.x * TW + col;
+  // neighbouring tiles share the cache lines at their row ends (rows of Nkx
+  // complex values are not line aligned): keep them on one XCD's L2
+  const int q = xcd_group(blockIdx.x, gridDim.x) * TW + col;
   const bool ok = q < ncols;
   const int s = ok ? q / Nkx : 0;
   const int kx = ok ? q - s * Nkx : 0;

@@ -54,6 +54,7 @@ struct StencilArgs {
   int periodic_z;        // triply periodic: uniform z nodes k hz, wrapped like x and y
   double hz;
   int dip_regular;       // the torque kernel is a plain ES kernel: records in the monopole layout
+  int first_kind;        // 1: the second (kd) record set only (e.g. force-kernel derivative records)
 };
 
 struct SpreadArgs {
@@ -105,6 +106,15 @@ void launch_tp_curl(const void* in, void* out, int Nx, int Ny, int Nz, double Lx
 bool fft_y_supported(int Ny);
 template <class T>
 void launch_fft_y(void* data, int nslices, int Nkx, int Ny, const void* tw, int dir, hipStream_t s);
+
+// Position gradient of the adjoint: per particle p (records of the force
+// kernel's derivatives, dipole layout), with G_u[j][i] = D_j-interpolation of
+// component i of grid gu and likewise G_v for gv,
+//   grad[p][j] = -sum_i (lam[p][i] G_u[j][i] + f[p][i] G_v[j][i]).
+template <class T, class Tio>
+void launch_interp_grad(const InterpArgs& A, const int4* sup, const void* rec, int rs, const KernelDesc& K,
+                        const void* gu, const void* gv, const void* lam, const void* f, void* grad,
+                        hipStream_t s);
 
 // Fourier solve of the triply periodic Stokes problem on the r2c spectrum
 // [c][kz][ky][kx] (kx in [0, Nx/2]), scaled by 1/(Nx Ny Nz) for the c2r

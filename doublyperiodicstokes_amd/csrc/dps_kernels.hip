@@ -89,7 +89,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
   A.pos[3 * p + 1] = y;
   A.pos[3 * p + 2] = z;
   const double hxhy = A.hx * A.hy;
-  for (int kind = 0; kind < (A.has_dip ? 2 : 1); ++kind) {
+  for (int kind = A.first_kind; kind < (A.has_dip ? 2 : 1); ++kind) {
     // deriv: the torque kernel's derivative records; a regular torque kernel
     // (spectral curl) keeps the monopole layout
     const bool deriv = kind && !A.dip_regular;
@@ -138,7 +138,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
           rec[o + q] = (T)ph;
           rec[o + W + q] = (T)(ph * qw);
         } else {
-          const double d = ph * es_dfac(v, K.betaw, a2);
+          const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
           rec[o + q] = (T)ph;
           rec[o + W + q] = (T)d;
           rec[o + 2 * W + q] = (T)(ph * qw);
@@ -188,7 +188,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
         rec[o + r] += (T)(ph * sa);                 // spread
         rec[o + W + r] += (T)(ph * qw * sa);        // interpolation
       } else {
-        const double d = ph * es_dfac(v, K.betaw, a2);
+        const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
         rec[o + r] += (T)(ph * sa);
         rec[o + W + r] += (T)(d * sb);
         rec[o + 2 * W + r] += (T)(ph * qw * sa);
@@ -612,6 +612,77 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
         ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[1] - d[3]));
       }
     }
+  }
+}
+
+// Position gradient of the adjoint (no counterpart in the reference): one
+// wavefront per particle, a lane per support point, the derivative records of
+// the force kernel in the dipole layout ([wx][wy][dx][dy][z][dz][z interp]
+// [dz interp]); both grids are read at each point and the two 3x3 gradient
+// tensors reduced over the wavefront.
+template <class T, class Tio>
+__global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* __restrict__ sup_g,
+                                                     const T* __restrict__ recs, int rs, KernelDesc K,
+                                                     const T* __restrict__ U, const T* __restrict__ V,
+                                                     const Tio* __restrict__ lam, const Tio* __restrict__ f,
+                                                     Tio* __restrict__ grad)
+{
+  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= A.N) return;
+  const int p = A.order[g];
+  const size_t cs = (size_t)A.Nz * A.Ny * A.Nx;
+  const size_t zs = (size_t)A.Ny * A.Nx;
+  const int4 sup = sup_g[p];
+  const T* rec = recs + (size_t)p * rs;
+  const int W = K.wzmax, nz = sup.w, wx = K.wx, wy = K.wy;
+  const T* wxr = rec;
+  const T* wyr = rec + wx;
+  const T* dxr = wyr + wy;
+  const T* dyr = dxr + wx;
+  const T* zr = dyr + wy;
+  const T* wzi = zr + 2 * W;
+  const T* dzi = zr + 3 * W;
+  const int ncol = wx * wy;
+  const int npts = ncol * nz;
+  const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
+  T du[9], dv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) du[i] = dv[i] = 0;
+  for (int t = lane; t < npts; t += 64) {
+    int col, a;
+    const int q = fdiv(t, ncol, incol, &col);
+    const int b = fdiv(col, wx, iwx, &a);
+    int x = sup.x + a, y = sup.y + b;
+    x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+    y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+    int zr_ = sup.z + q;
+    if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+    const size_t idx = (size_t)zr_ * zs + (size_t)y * A.Nx + x;
+    const T gx = dxr[a] * wyr[b] * wzi[q];
+    const T gy = wxr[a] * dyr[b] * wzi[q];
+    const T gz = wxr[a] * wyr[b] * dzi[q];
+    const T u0 = U[idx], u1 = U[idx + cs], u2 = U[idx + 2 * cs];
+    const T v0 = V[idx], v1 = V[idx + cs], v2 = V[idx + 2 * cs];
+    du[0] += gx * u0; du[1] += gx * u1; du[2] += gx * u2;
+    du[3] += gy * u0; du[4] += gy * u1; du[5] += gy * u2;
+    du[6] += gz * u0; du[7] += gz * u1; du[8] += gz * u2;
+    dv[0] += gx * v0; dv[1] += gx * v1; dv[2] += gx * v2;
+    dv[3] += gy * v0; dv[4] += gy * v1; dv[5] += gy * v2;
+    dv[6] += gz * v0; dv[7] += gz * v1; dv[8] += gz * v2;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    du[i] = wave_sum(du[i]);
+    dv[i] = wave_sum(dv[i]);
+  }
+  if (lane == 0) {
+    const double l0 = (double)lam[3 * p], l1 = (double)lam[3 * p + 1], l2 = (double)lam[3 * p + 2];
+    const double f0 = (double)f[3 * p], f1 = (double)f[3 * p + 1], f2 = (double)f[3 * p + 2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      grad[3 * p + j] = (Tio)(-(l0 * (double)du[3 * j] + l1 * (double)du[3 * j + 1] + l2 * (double)du[3 * j + 2]) -
+                              (f0 * (double)dv[3 * j] + f1 * (double)dv[3 * j + 1] + f2 * (double)dv[3 * j + 2]));
   }
 }
 
@@ -1220,6 +1291,18 @@ void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream
   }
 }
 
+template <class T, class Tio>
+void launch_interp_grad(const InterpArgs& A, const int4* sup, const void* rec, int rs, const KernelDesc& K,
+                        const void* gu, const void* gv, const void* lam, const void* f, void* grad,
+                        hipStream_t s)
+{
+  if (A.N == 0) return;
+  const int waves_per_block = 256 / 64;
+  hipLaunchKernelGGL((k_interp_grad<T, Tio>), dim3((A.N + waves_per_block - 1) / waves_per_block), dim3(256), 0,
+                     s, A, sup, (const T*)rec, rs, K, (const T*)gu, (const T*)gv, (const Tio*)lam,
+                     (const Tio*)f, (Tio*)grad);
+}
+
 template <class T, class Tout>
 void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s)
 {
@@ -1295,6 +1378,12 @@ void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipS
   template void launch_spread<T, double>(const SpreadArgs&, const void*, const void*, hipStream_t); \
   template void launch_interp<T, float>(const InterpArgs&, void*, void*, hipStream_t);      \
   template void launch_interp<T, double>(const InterpArgs&, void*, void*, hipStream_t);     \
+  template void launch_interp_grad<T, float>(const InterpArgs&, const int4*, const void*, int,       \
+                                             const KernelDesc&, const void*, const void*,             \
+                                             const void*, const void*, void*, hipStream_t);           \
+  template void launch_interp_grad<T, double>(const InterpArgs&, const int4*, const void*, int,      \
+                                              const KernelDesc&, const void*, const void*,            \
+                                              const void*, const void*, void*, hipStream_t);          \
   template void launch_zfwd<T>(const ZArgs&, const void*, void*, hipStream_t);              \
   template void launch_zinv<T>(const ZArgs&, const void*, void*, hipStream_t);              \
   template void launch_bvp<T>(const BvpArgs&, const void*, void*, hipStream_t);             \

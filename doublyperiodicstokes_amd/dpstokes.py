@@ -235,6 +235,30 @@ class DPStokes:
                                   _dtype_code(dt), _stream()))
         return U, W
 
+    def mdot_adjoint_device(self, forces, grad_velocities, torques=None, grad_angular=None, positions=True):
+        """Adjoint of Mdot at the current positions (dps_mdot_adjoint): returns
+        (dL/dF, dL/dT or None, dL/dx or None) for L = <gU, U> + <gW, W>.
+        dL/dT comes with grad_angular; position gradients need forces only
+        (no torques, no grad_angular)."""
+        self._need()
+        torch = _torch()
+        gU = self._to_device(grad_velocities, "grad_velocities", 3 * self.N)
+        dt = gU.dtype
+        F = self._to_device(forces, "forces", 3 * self.N).to(dt)
+        T = gW = None
+        if torques is not None and self.has_torque:
+            T = self._to_device(torques, "torques", 3 * self.N).to(dt)
+        if grad_angular is not None and self.has_torque:
+            gW = self._to_device(grad_angular, "grad_angular", 3 * self.N).to(dt)
+        gF = torch.empty(3 * self.N, dtype=dt, device=self.device)
+        gT = torch.empty(3 * self.N, dtype=dt, device=self.device) if gW is not None else None
+        gX = torch.empty(3 * self.N, dtype=dt, device=self.device) if positions else None
+        vp = ctypes.c_void_p
+        ptr = lambda t: vp(t.data_ptr()) if t is not None else None
+        N.check(N.load().dps_mdot_adjoint(self._h, ptr(F), ptr(T), ptr(gU), ptr(gW), ptr(gF), ptr(gT), ptr(gX),
+                                          _dtype_code(dt), _stream()))
+        return gF, gT, gX
+
     def Mdot(self, forces, torques=None, velocities=None, angularVelocities=None):
         """Velocities (and angular velocities when torques are given).
 

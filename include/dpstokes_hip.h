@@ -123,6 +123,27 @@ int dps_update_positions(dps_handle* h, void* positions, const void* velocities,
 int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* velocities,
              void* angular, int dtype, void* stream);
 
+/* Adjoint of dps_mdot at the current positions (no counterpart in the
+   reference; the differentiable form of the path).  With
+   L = <grad_velocities, U> + <grad_angular, W> and (U, W) = Mdot(forces,
+   torques):
+     grad_forces, grad_torques = dL/dF, dL/dT = Mdot(grad_velocities,
+       grad_angular), the solve taken as self-adjoint (exact when triply
+       periodic; doubly periodic, up to the discretisation's asymmetry);
+       grad_torques needs grad_forces and grad_angular (zeros when W does not
+       enter L);
+     grad_positions = dL/dx (3N), forces only: torques, grad_angular and
+       grad_torques must be NULL.  It is
+       -sum_i gU[p,i] D_j u_i(x_p) - sum_i F[p,i] D_j v_i(x_p),
+       with u the velocity grid of the forces, v that of grad_velocities and
+       D_j interpolation with the j-th derivative of the force kernel (zero
+       where an offset is clamped); two spreads, two solves and one gradient
+       interpolation.
+   Outputs may be NULL (not computed).  All arrays use `dtype`. */
+int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, const void* grad_velocities,
+                     const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
+                     int dtype, void* stream);
+
 /* Stage entry points (for stage-level parity tests and the adjoint):
    spread forces (+torque curl) to the real grid, returned in the reference's
    interior layout (Ny, Nx, Nz, 3) FP64; solve grid -> velocity grid (same

@@ -729,6 +729,14 @@ class OracleDPStokes:
         om[2::3] = -1 / 2 * (dxv[1::3] - dyv[0::3])
         return v, om
 
+    def interp_force_kernel(self, kind, uG):
+        """Interpolate uG with the force kernel (kind 0) or its x/y/z
+        derivative (kind 1/2/3) as a function of the particle position (zero
+        where an offset was clamped), folds and ghosts as for the force
+        kernel."""
+        return self._interp(kind + 3 if kind else 0, self._loc, self.mono, uG, self.BCs,
+                            self._ext_buffer(self._loc))
+
     def mdot(self, forces, torques=None):
         """Linear and angular velocities (FCM.py:334-441)."""
         forces = np.asarray(forces, dtype=np.float64)
@@ -741,6 +749,35 @@ class OracleDPStokes:
         uG = self.solve(F, self.tgrid)
         self.uG = uG
         return self.interpolate(uG, tp, self.utG)
+
+
+def adjoint(o, forces, lam_u, lam_w=None, torques=None):
+    """Adjoint of Mdot for either oracle (no counterpart in the reference).
+    With L = <lam_u, U> + <lam_w, W> and (U, W) = Mdot(F, T) at the current
+    positions:
+      dL/dF, dL/dT = Mdot(lam_u, lam_w)   (the solve taken as self-adjoint)
+      dL/dx_p = -sum_i lam_u[p,i] D_j u_i(x_p) - sum_i F[p,i] D_j v_i(x_p)
+    (forces only; None with torques), where u is the velocity grid of F, v
+    the velocity grid of lam_u, and D_j interpolation with the j-th
+    derivative of the force kernel: a weight phi(x_g - x_p) changes with x_p
+    as minus the kernel derivative, and ghost folds and copies keep the force
+    kernel's factors because the ghost nodes do not move."""
+    forces = np.asarray(forces, dtype=np.float64)
+    lam_u = np.asarray(lam_u, dtype=np.float64)
+    gF, gT = o.mdot(lam_u, lam_w)
+    if (torques is not None and np.size(torques) > 0) or (lam_w is not None and np.size(lam_w) > 0):
+        return gF, gT, None
+    uG = o.solve(o.spread_forces(forces, None))
+    vG = o.solve(o.spread_forces(lam_u, None))
+    N = o.N
+    lam = lam_u.reshape(N, 3)
+    F3 = forces.reshape(N, 3)
+    gx = np.empty((N, 3))
+    for j in (1, 2, 3):
+        du = o.interp_force_kernel(j, uG).reshape(N, 3)
+        dv = o.interp_force_kernel(j, vG).reshape(N, 3)
+        gx[:, j - 1] = -(lam * du).sum(1) - (F3 * dv).sum(1)
+    return gF, gT, gx.ravel()
 
 
 def num_threads():

@@ -193,14 +193,19 @@ int orc_locate(int nP, const double* xP, double hx, double hy, double Lx, double
   return 0;
 }
 
-/* ES kernel (kind 0) or its x/y/z derivative (kind 1/2/3); include/ESKernels.h:8-46 */
+/* ES kernel (kind 0) or its x/y/z derivative (kind 1/2/3); include/ESKernels.h:8-46.
+   Kinds 4/5/6 (no counterpart in the reference; the adjoint with respect to
+   positions): the x/y/z derivative of the weight as a function of the
+   particle position, which is zero where the offset was clamped (the weight
+   is then the constant phi(thresh)); the caller passes thresh as alpha. */
 static inline double kern3(int kind, const double x[3], double beta, double alpha)
 {
   const double alpha2 = alpha * alpha;
+  if (kind >= 4 && fabs(x[kind - 4]) >= alpha) return 0.0;
   const double e = exp(beta * (sqrt(1 - x[0] * x[0] / alpha2) + sqrt(1 - x[1] * x[1] / alpha2) +
                                sqrt(1 - x[2] * x[2] / alpha2) - 3));
   if (kind == 0) return e;
-  const int c = kind - 1;
+  const int c = (kind - 1) % 3;
   return -beta * x[c] * e / (alpha2 * sqrt(1 - x[c] * x[c] / (alpha2)));
 }
 

@@ -188,6 +188,25 @@ class OracleTPStokes:
             self.utG = np.real(np.fft.ifftn(self.curl_hat(U, Dx, Dy, Dz), axes=(0, 1, 2)))
         return np.real(np.fft.ifftn(U, axes=(0, 1, 2)))
 
+    def interp_force_kernel(self, kind, uG):
+        """Interpolate uG with the force kernel (kind 0) or its x/y/z
+        derivative (kind 1/2/3) as a function of the particle position: zero
+        where an offset was clamped (the weight is then the constant
+        phi(thresh))."""
+        if kind == 0:
+            return self._interp(0, self._loc, self.mono, uG).ravel()
+        (ix, ox), (iy, oy), (iz, oz) = self._loc
+        o = (ox, oy, oz)[kind - 1]
+        keep = (np.abs(o) < self.mono.thresh).astype(np.float64)
+        W = _kern3(kind, ox, oy, oz, self.mono.betaw, self.mono.alpha) / self.mono.norm ** 3
+        shape = [(1, 1, 1, -1), (1, 1, -1, 1), (1, -1, 1, 1)][kind - 1]
+        W = W * keep.reshape((keep.shape[0],) + shape[1:]) * (self.hx * self.hy * self.hz)
+        with np.errstate(invalid="ignore"):
+            W = np.nan_to_num(W, nan=0.0, posinf=0.0, neginf=0.0)
+        g = self._flat_index(self._loc)
+        U = uG.reshape(-1, 3)
+        return np.stack([np.sum(U[g, d] * W, axis=(1, 2, 3)) for d in range(3)], 1).ravel()
+
     def mdot(self, forces, torques=None):
         forces = np.asarray(forces, dtype=np.float64)
         tp = self.has_torque and torques is not None and np.size(torques) > 0

@@ -242,7 +242,7 @@ void cc_stencil(int N, const double* pos, double hx, double hy, double betaw, do
                 int wzmax, int* sup /*4N*/, double* wxo, double* dxo, double* wyo, double* dyo,
                 double* wzo, double* dzo, double* wzio, double* dzio)
 {
-  KernelDesc K;
+  KernelDesc K{};
   K.betaw = betaw; K.alpha = alpha; K.thresh = thresh; K.inv_norm3 = inv_norm3;
   K.wx = wx; K.wy = wy; K.nzeff = nzeff; K.ext_up = 0; K.wzmax = wzmax;
   for (int p = 0; p < N; ++p) {
