@@ -28,7 +28,8 @@ DPS_F32 = 0
 DPS_F64 = 1
 
 EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions", "dps_mdot",
-           "dps_mdot_adjoint", "dps_stage_spread",
+           "dps_mdot_adjoint", "dps_batch_create", "dps_batch_destroy", "dps_batch_mdot", "dps_batch_tune",
+           "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
            "dps_get_timings", "dps_get_precision", "dps_last_error")
 
@@ -72,6 +73,11 @@ def load():
     L.dps_update_positions.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
     L.dps_mdot.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_mdot_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
+    pp = ctypes.POINTER(vp)
+    L.dps_batch_create.argtypes = [pp, ctypes.c_int, ctypes.c_int, pp]
+    L.dps_batch_destroy.argtypes = [vp]
+    L.dps_batch_mdot.argtypes = [vp, pp, pp, pp, pp, ctypes.c_int, vp]
+    L.dps_batch_tune.argtypes = [vp, pp, pp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
     L.dps_stage_spread.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp]
     L.dps_stage_solve.argtypes = [vp, vp, vp, vp]
     L.dps_stage_interp.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]

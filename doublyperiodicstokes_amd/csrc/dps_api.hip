@@ -156,6 +156,40 @@ struct dps_handle {
   }
 };
 
+// A set of independent solvers run as one: every instance's Mdot captured,
+// forked over a few streams, into one HIP graph that is replayed while the
+// argument pointers stay the same.
+struct dps_batch {
+  std::vector<dps_handle*> hs;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr;
+  hipStream_t cap = nullptr;   // capture origin (the caller's stream may be the null stream)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;   // pointers and dtype the graph was captured with
+  void drop_graph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+  void release()
+  {
+    drop_graph();
+    for (auto e : ev_join) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (auto st : streams) (void)hipStreamDestroy(st);
+    if (cap) (void)hipStreamDestroy(cap);
+    cap = nullptr;
+    ev_join.clear();
+    streams.clear();
+    ev_fork = nullptr;
+  }
+};
+
 namespace {
 
 int setup_kernel(dps_handle* h, int w, double alpha, double beta, bool dipole, KernelDesc* K)
@@ -1169,6 +1203,162 @@ int dps_stage_interp(dps_handle* h, const double* grid_in, void* velocities, voi
   else launch_ref_to_grid<double>(grid_in, h->d_grid, h->Nx, h->Ny, h->Nz, s);
   HIPCHK(hipGetLastError());
   return do_interp(h, velocities, angular, dtype, s);
+}
+
+int dps_batch_create(dps_handle* const* handles, int n, int nstreams, dps_batch** out)
+{
+  if (!out) return fail(DPS_ERR_ARG, "null output batch");
+  *out = nullptr;
+  if (!handles || n < 1) return fail(DPS_ERR_ARG, "a batch needs at least one handle");
+  if (nstreams < 1 || nstreams > 16) return fail(DPS_ERR_ARG, "nstreams must be in [1, 16]");
+  for (int i = 0; i < n; ++i)
+    if (!handles[i]) return fail(DPS_ERR_ARG, "null handle in batch");
+  dps_batch* b = new dps_batch();
+  b->hs.assign(handles, handles + n);
+  for (int j = 0; j < nstreams; ++j) {
+    hipStream_t st = nullptr;
+    hipEvent_t e = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      if (st) (void)hipStreamDestroy(st);
+      b->release();
+      delete b;
+      return fail(DPS_ERR_HIP, "stream or event creation failed");
+    }
+    b->streams.push_back(st);
+    b->ev_join.push_back(e);
+  }
+  if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+    b->release();
+    delete b;
+    return fail(DPS_ERR_HIP, "event or stream creation failed");
+  }
+  *out = b;
+  return DPS_OK;
+}
+
+int dps_batch_destroy(dps_batch* b)
+{
+  if (!b) return DPS_OK;
+  b->release();
+  delete b;
+  return DPS_OK;
+}
+
+int dps_batch_mdot(dps_batch* b, const void* const* forces, const void* const* torques, void* const* velocities,
+                   void* const* angular, int dtype, void* stream)
+{
+  if (!b) return fail(DPS_ERR_STATE, "null batch");
+  if (!forces || !velocities) return fail(DPS_ERR_ARG, "null forces or velocities array");
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  const int n = (int)b->hs.size();
+  std::vector<const void*> key;
+  key.reserve(4 * n + 1);
+  for (int i = 0; i < n; ++i) {
+    int rc = check_handle(b->hs[i], true);
+    if (rc) return rc;
+    const dps_handle* h = b->hs[i];
+    if (h->N > 0 && (!forces[i] || !velocities[i])) return fail(DPS_ERR_ARG, "null forces or velocities");
+    const void* t = torques ? torques[i] : nullptr;
+    if (t && !h->has_dip) return fail(DPS_ERR_ARG, "torques given but no dipole kernel (w_d = 0)");
+    key.push_back(forces[i]);
+    key.push_back(t);
+    key.push_back(velocities[i]);
+    key.push_back(angular ? angular[i] : nullptr);
+  }
+  key.push_back((const void*)(intptr_t)(dtype + 1));
+  hipStream_t s = (hipStream_t)stream;
+  if (!b->exec || key != b->key) {
+    b->drop_graph();
+    // capturing records no work: capture on the batch's own stream, launch on
+    // the caller's
+    hipStream_t c = b->cap;
+    HIPCHK(hipStreamBeginCapture(c, hipStreamCaptureModeThreadLocal));
+    int rc = DPS_OK;
+    hipError_t e = hipEventRecord(b->ev_fork, c);
+    const int ns = (int)b->streams.size();
+    for (int j = 0; j < ns && e == hipSuccess; ++j) e = hipStreamWaitEvent(b->streams[j], b->ev_fork, 0);
+    for (int i = 0; i < n && e == hipSuccess && !rc; ++i) {
+      dps_handle* h = b->hs[i];
+      const bool p = h->prof;
+      h->prof = false;   // no timing events inside the graph
+      rc = dps_mdot(h, forces[i], torques ? torques[i] : nullptr, velocities[i], angular ? angular[i] : nullptr,
+                    dtype, (void*)b->streams[i % ns]);
+      h->prof = p;
+      h->have_timings = false;
+    }
+    for (int j = 0; j < ns && e == hipSuccess; ++j) {
+      e = hipEventRecord(b->ev_join[j], b->streams[j]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(c, b->ev_join[j], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c, &g);
+    if (rc || e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc) return rc;
+      return fail(DPS_ERR_HIP, std::string("batch capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    b->graph = g;
+    const hipError_t e3 = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) {
+      b->drop_graph();
+      return fail(DPS_ERR_HIP, std::string("batch graph instantiation failed: ") + hipGetErrorString(e3));
+    }
+    b->key = key;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, s));
+  return DPS_OK;
+}
+
+int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* torques, int dtype, void* stream,
+                   int* chunk_out)
+{
+  if (!b) return fail(DPS_ERR_STATE, "null batch");
+  if (!forces) return fail(DPS_ERR_ARG, "null forces array");
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(DPS_ERR_HIP, "event creation failed");
+  }
+  static const int cands[] = {4, 8, 12, 16, 24, 32, 48};
+  int rc = DPS_OK;
+  for (size_t i = 0; i < b->hs.size() && !rc; ++i) {
+    dps_handle* h = b->hs[i];
+    if ((rc = check_handle(h, true))) break;
+    const int keep = h->ZC;
+    int best = keep;
+    float best_ms = 1e30f;
+    for (int zc : cands) {
+      if (zc > h->Nz) continue;
+      SpreadArgs S = spread_args(h);
+      S.ZC = zc;
+      if (spread_lds_bytes(S, h->tsz) > 65536) continue;
+      h->ZC = zc;
+      const void* t = (torques && h->has_dip) ? torques[i] : nullptr;
+      // one untimed call, then the spreading of three calls timed together
+      if ((rc = do_spread(h, forces[i], t, dtype, s))) break;
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(DPS_ERR_HIP, "event record failed"); break; }
+      for (int r = 0; r < 3 && !rc; ++r) rc = do_spread(h, forces[i], t, dtype, s);
+      if (rc) break;
+      float ms = 0;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(DPS_ERR_HIP, "timing failed");
+        break;
+      }
+      if (ms < best_ms) { best_ms = ms; best = zc; }
+    }
+    h->ZC = best;
+    if (chunk_out) chunk_out[i] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->drop_graph();   // launch shapes changed
+  return rc;
 }
 
 int dps_get_info(const dps_handle* h, int* out)

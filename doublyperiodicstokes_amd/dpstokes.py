@@ -352,3 +352,73 @@ class DPStokes:
         N.check(N.load().dps_get_timings(self._h, out))
         keys = ("locate", "spread", "fft_fwd", "z_fwd", "bvp", "z_inv", "fft_inv", "interp")
         return dict(zip(keys, list(out)))
+
+
+class DPStokesBatch:
+    """Many independent solvers run as one (dps_batch_*): every instance's
+    Mdot captured into one HIP graph, forked over ``nstreams`` streams, and
+    replayed while the same input tensors are passed.  The solvers stay
+    owned by the caller and must outlive the batch."""
+
+    def __init__(self, solvers, nstreams=4):
+        for s in solvers:
+            s._need()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        b = ctypes.c_void_p()
+        N.check(N.load().dps_batch_create(arr, len(self.solvers), int(nstreams), ctypes.byref(b)))
+        self._b = b
+        self._out = {}
+
+    def clear(self):
+        if getattr(self, "_b", None) is not None and self._b.value:
+            N.load().dps_batch_destroy(self._b)
+        self._b = None
+
+    def __del__(self):
+        try:
+            self.clear()
+        except Exception:
+            pass
+
+    def _inputs(self, forces, torques):
+        if len(forces) != len(self.solvers):
+            raise ValueError("need one force array per instance")
+        F = [s._to_device(f, "forces", 3 * s.N) for s, f in zip(self.solvers, forces)]
+        dt = F[0].dtype
+        F = [f.to(dt) for f in F]
+        T = [None] * len(F)
+        if torques is not None:
+            if len(torques) != len(self.solvers):
+                raise ValueError("need one torque array (or None) per instance")
+            T = [s._to_device(t, "torques", 3 * s.N).to(dt) if (t is not None and s.has_torque) else None
+                 for s, t in zip(self.solvers, torques)]
+        return F, T, dt
+
+    @staticmethod
+    def _ptrs(ts):
+        return (ctypes.c_void_p * len(ts))(*[t.data_ptr() if t is not None else None for t in ts])
+
+    def tune(self, forces, torques=None):
+        """Autotune the spreading z chunk of every instance; returns the chunks."""
+        F, T, dt = self._inputs(forces, torques)
+        out = (ctypes.c_int * len(F))()
+        N.check(N.load().dps_batch_tune(self._b, self._ptrs(F), self._ptrs(T), _dtype_code(dt), _stream(), out))
+        return list(out)
+
+    def mdot_device(self, forces, torques=None):
+        """Velocities (and angular velocities where torques are given) of every
+        instance, as lists of device tensors owned by the batch (overwritten by
+        the next call)."""
+        torch = _torch()
+        F, T, dt = self._inputs(forces, torques)
+        key = (dt, tuple(t is not None for t in T))
+        if key not in self._out:
+            dev = self.solvers[0].device
+            self._out[key] = ([torch.empty(3 * s.N, dtype=dt, device=dev) for s in self.solvers],
+                              [torch.empty(3 * s.N, dtype=dt, device=dev) if t is not None else None
+                               for s, t in zip(self.solvers, T)])
+        U, W = self._out[key]
+        N.check(N.load().dps_batch_mdot(self._b, self._ptrs(F), self._ptrs(T), self._ptrs(U), self._ptrs(W),
+                                        _dtype_code(dt), _stream()))
+        return U, W

@@ -89,6 +89,7 @@ typedef struct dps_params {
 } dps_params;
 
 typedef struct dps_handle dps_handle;
+typedef struct dps_batch dps_batch;
 
 /* Create a solver for `number_particles` particles (DPStokesGlue::initialize,
    uammd_wrapper.cu:119-133).  Allocates all device memory and FFT plans on the
@@ -155,6 +156,26 @@ int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int
 int dps_stage_solve(dps_handle* h, const double* grid_in, double* grid_out, void* stream);
 int dps_stage_interp(dps_handle* h, const double* grid_in, void* velocities, void* angular,
                      int dtype, void* stream);
+
+/* Batched form (no counterpart in the reference; many independent
+   suspensions as one launch).  A batch groups existing handles (each with its
+   own box, grid and particle count; they stay owned by the caller and must
+   outlive the batch).  dps_batch_mdot runs every instance's dps_mdot: the
+   first call captures them, forked over `nstreams` internal streams (default
+   priority), into one HIP graph, and later calls with the same pointers and
+   dtype replay it with one launch (other pointers re-capture).  Each array
+   argument holds one pointer per instance; torques / angular (the arrays or
+   their entries) may be NULL.  Position updates of the instances (same
+   buffers) are seen by replays.  dps_batch_tune times the spreading z-chunk
+   candidates of every instance on its current positions and inputs, keeps
+   the fastest (chunk_out: one int per instance, may be NULL) and drops the
+   captured graph. */
+int dps_batch_create(dps_handle* const* handles, int n, int nstreams, dps_batch** out);
+int dps_batch_destroy(dps_batch* b);
+int dps_batch_mdot(dps_batch* b, const void* const* forces, const void* const* torques, void* const* velocities,
+                   void* const* angular, int dtype, void* stream);
+int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* torques, int dtype, void* stream,
+                   int* chunk_out);
 
 /* Query derived sizes: nx, ny, nz, number of particles, Nzeff of the monopole
    and dipole z-extended grids, z support widths.  `out` holds 8 ints. */
