@@ -140,6 +140,88 @@ def cpu_baseline(wl, pos, F, T, budget_s, reg=False):
     return n / el, n, el, o.num_threads()
 
 
+def batch_workload(n_inst=64, seed=0):
+    """64 independent bottom-wall suspensions: N log-uniform in [256, 32768],
+    Lx = Ly for area fraction 0.3 (a = 1), box height log-uniform in [12, 48],
+    h = 0.5 (w = 6, alpha = 1.5); Nx = Ny the smallest 2^k or 3*2^k with
+    Nx h >= L (then L = Nx h); the z grid from the reference's bottom-wall rule
+    for the height; x, y uniform, z uniform in [1.5, 0.8 height]; forces and
+    torques iid N(0, 1).  numpy RandomState(seed), draw order: all N, all
+    heights, then per instance x, y, z, F, T."""
+    from doublyperiodicstokes_amd.gridconfig import configure_grid_and_kernels_z
+    rng = np.random.RandomState(seed)
+    Ns = np.round(np.exp(rng.uniform(np.log(256), np.log(32768), n_inst))).astype(int)
+    heights = np.exp(rng.uniform(np.log(12), np.log(48), n_inst))
+    h = 0.5
+    out = []
+    for N, H in zip(Ns, heights):
+        L = np.sqrt(N * np.pi / 0.3)
+        n = min(c for c in [2 ** k * m for k in range(4, 12) for m in (2, 3)] if c * h >= L)
+        L = n * h
+        Lz, hz, nz, a = configure_grid_and_kernels_z(0.0, H, h, np.array([6]), np.array([6]), "DPBW")
+        x = rng.uniform(0, L, N)
+        y = rng.uniform(0, L, N)
+        z = rng.uniform(1.5, 0.8 * H, N)
+        F = rng.normal(size=3 * N)
+        T = rng.normal(size=3 * N)
+        out.append(dict(N=int(N), L=float(L), zmax=float(a + Lz), zmin=float(a), n=int(n), nz=int(nz),
+                        pos=np.stack([x, y, z], 1).ravel(), F=F, T=T))
+    return out
+
+
+def bench_batch(args):
+    """Batched Mdot of the 64-instance workload: one captured graph per step."""
+    import torch
+    from doublyperiodicstokes_amd import DPStokes, DPStokesBatch, StokesParameters
+    insts = batch_workload()
+    dev = torch.device("cuda", 0)
+    solvers, F, T = [], [], []
+    t0 = time.perf_counter()
+    for d in insts:
+        par = StokesParameters(viscosity=1 / (6 * np.pi), Lx=d["L"], Ly=d["L"], zmin=d["zmin"], zmax=d["zmax"],
+                               w=6, w_d=6, beta=1.785 * 6, beta_d=1.785 * 6, nx=d["n"], ny=d["n"], nz=d["nz"],
+                               mode="bottom", precision="mixed")
+        s = DPStokes()
+        s.initialize(par, d["N"])
+        s.setPositions(d["pos"])
+        solvers.append(s)
+        F.append(torch.from_numpy(d["F"].astype(np.float32)).to(dev))
+        T.append(torch.from_numpy(d["T"].astype(np.float32)).to(dev))
+    setup_s = time.perf_counter() - t0
+    b = DPStokesBatch(solvers, nstreams=4)
+    chunks = b.tune(F, T)
+    ms_graph = time_steps(torch, lambda: b.mdot_device(F, T), args.steps, args.warmup)
+
+    def eager():
+        for s, f, t in zip(solvers, F, T):
+            s.mdot_device(f, t)
+    ms_eager = time_steps(torch, eager, max(3, args.steps // 2), 1)
+    ntot = sum(d["N"] for d in insts)
+    out = {
+        "metric": "batched Mdot/s: 64 independent bottom-wall instances (forces + torques) per step",
+        "value": 1000.0 / ms_graph,
+        "unit": "batches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_graph,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grids + f64 BVP",
+        "data": "synthetic",
+        "config": {"workload": "64 bottom-wall instances, N log-uniform [256, 32768] (total %d), height "
+                               "log-uniform [12, 48], h = 0.5, w = w_d = 6, mixed" % ntot},
+        "particles_per_s": ntot * 1000.0 / ms_graph,
+        "eager_ms_per_step": ms_eager,
+        "graph_speedup_vs_eager": ms_eager / ms_graph,
+        "grids": sorted(set((d["n"], d["nz"]) for d in insts)),
+        "tuned_z_chunks": chunks,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,11 +235,16 @@ def main():
                     help="torques through the regular kernel with spectral curls (useRegKernel)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy of this many MiB (profiler byte-counter calibration)")
+    ap.add_argument("--batch", action="store_true",
+                    help="the 64-instance batched workload instead of the single slit instance")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        bench_batch(args)
+        return
     import torch
     wl = WORKLOADS[args.workload]
     geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
