@@ -7,8 +7,9 @@
 The backward pass is dps_mdot_adjoint: force and torque gradients are one
 Mdot of the output gradients (the solve taken as self-adjoint), and position
 gradients come from derivative-kernel interpolation of the velocity grids of
-the forces and of the output gradient.  Position gradients are available
-when no torques are applied.
+the inputs and of the output gradients (second derivatives of the torque
+kernel for the torque terms).  Position gradients with torques need the
+derivative torque kernels (not useRegKernel).
 """
 import torch
 
@@ -36,11 +37,10 @@ class _Mdot(torch.autograd.Function):
         need_x, need_f, need_t = ctx.needs_input_grad[1], ctx.needs_input_grad[2], ctx.needs_input_grad[3]
         gF = gT = gX = None
         if ctx.has_t:
-            if need_x:
-                raise NotImplementedError("position gradients are available for forces only (no torques)")
             if gW is None or gW.numel() == 0:
                 gW = torch.zeros_like(gU)
-            gF, gT, _ = solver.mdot_adjoint_device(forces, gU, grad_angular=gW.contiguous(), positions=False)
+            gF, gT, gX = solver.mdot_adjoint_device(forces, gU, torques=torques, grad_angular=gW.contiguous(),
+                                                    positions=need_x)
         else:
             gF, _, gX = solver.mdot_adjoint_device(forces, gU, positions=need_x)
         return (None,

@@ -85,6 +85,8 @@ struct dps_handle {
   void* d_rec_md = nullptr;
   int4* d_zrange_md = nullptr;
   int rs_md = 0;
+  void* d_rec_d2 = nullptr;   // torque-kernel second-derivative records
+  int rs_d2 = 0;
   double *d_zext_m = nullptr, *d_zw_m = nullptr, *d_zext_d = nullptr, *d_zw_d = nullptr;
   double *d_fA_m = nullptr, *d_fB_m = nullptr, *d_fA_d = nullptr, *d_fB_d = nullptr;
   int *d_kt_m = nullptr, *d_kt_d = nullptr;
@@ -933,6 +935,10 @@ int ensure_adjoint(dps_handle* h)
   if ((rc = h->alloc((unsigned char**)&h->d_rec_md, (size_t)h->N * h->rs_md * h->tsz))) return rc;
   if ((rc = h->alloc(&h->d_zrange_md, h->N))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_gridA, ng * h->tsz))) return rc;
+  if (h->has_dip && !h->reg) {
+    h->rs_d2 = 4 * h->kd.wx + 4 * h->kd.wy + 4 * h->kd.wzmax;
+    if ((rc = h->alloc((unsigned char**)&h->d_rec_d2, (size_t)h->N * h->rs_d2 * h->tsz))) return rc;
+  }
   return DPS_OK;
 }
 
@@ -1109,28 +1115,30 @@ int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, con
   if (h->N > 0 && !grad_velocities) return fail(DPS_ERR_ARG, "null grad_velocities");
   if ((grad_angular || grad_torques || torques) && !h->has_dip)
     return fail(DPS_ERR_ARG, "torque terms given but no dipole kernel (w_d = 0)");
-  if (grad_torques && (!grad_forces || !grad_angular))
-    return fail(DPS_ERR_ARG, "grad_torques needs grad_forces and grad_angular (zeros if W is unused)");
+  if (grad_torques && !grad_forces) return fail(DPS_ERR_ARG, "grad_torques needs grad_forces");
   if (!grad_positions) {
+    if (grad_torques && !grad_angular)
+      return fail(DPS_ERR_ARG, "grad_torques without positions needs grad_angular (zeros if W is unused)");
     // the solve is taken as self-adjoint: [dL/dF; dL/dT] = M [gU; gW]
     if (!grad_forces) return DPS_OK;
     return dps_mdot(h, grad_velocities, grad_angular, grad_forces, grad_torques, dtype, stream);
   }
-  if (torques || grad_angular || grad_torques)
-    return fail(DPS_ERR_ARG, "position gradients are available for forces only (no torque terms)");
+  const bool tterms = torques || grad_angular;
+  if ((tterms || grad_torques) && h->reg)
+    return fail(DPS_ERR_ARG, "position gradients with torques need the derivative torque kernels");
   if (h->N > 0 && !forces) return fail(DPS_ERR_ARG, "null forces");
   if ((rc = ensure_adjoint(h))) return rc;
   hipStream_t s = (hipStream_t)stream;
   const bool p = h->prof;
   h->prof = false;
   rc = locate_force_derivatives(h, s);
-  // u = L S F into d_gridA, v = L S gU into d_grid
-  if (!rc) rc = do_spread(h, forces, nullptr, dtype, s);
+  // u = velocity grid of (F, T) into d_gridA, v = that of (gU, gW) into d_grid
+  if (!rc) rc = do_spread(h, forces, torques, dtype, s);
   if (!rc) rc = do_solve(h, s, 1);
   std::swap(h->d_grid, h->d_gridA);
-  if (!rc) rc = do_spread(h, grad_velocities, nullptr, dtype, s);
+  if (!rc) rc = do_spread(h, grad_velocities, grad_angular, dtype, s);
   if (!rc) rc = do_solve(h, s, 1);
-  if (!rc && grad_forces) rc = do_interp(h, grad_forces, nullptr, dtype, s);
+  if (!rc && grad_forces) rc = do_interp(h, grad_forces, grad_torques, dtype, s);
   if (!rc) {
     const InterpArgs A = interp_args(h);
     const KernelDesc K = force_deriv_kernel(h);
@@ -1148,6 +1156,28 @@ int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, con
       else
         launch_interp_grad<double, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
                                            grad_velocities, forces, grad_positions, s);
+    }
+    if (tterms) {
+      const StencilArgs S = stencil_args(h);
+      const void* mu = grad_angular;
+      const void* tq = torques;
+      if (mixed(h)) {
+        launch_stencil_d2<float>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
+        if (dtype == DPS_F32)
+          launch_interp_grad_t<float, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                             mu, tq, grad_positions, s);
+        else
+          launch_interp_grad_t<float, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                              mu, tq, grad_positions, s);
+      } else {
+        launch_stencil_d2<double>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
+        if (dtype == DPS_F32)
+          launch_interp_grad_t<double, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                              mu, tq, grad_positions, s);
+        else
+          launch_interp_grad_t<double, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA,
+                                               h->d_grid, mu, tq, grad_positions, s);
+      }
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail(DPS_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));

@@ -116,6 +116,16 @@ void launch_interp_grad(const InterpArgs& A, const int4* sup, const void* rec, i
                         const void* gu, const void* gv, const void* lam, const void* f, void* grad,
                         hipStream_t s);
 
+// Torque-kernel records of the position adjoint (4 wx + 4 wy + 4 wzmax words
+// per particle, on the torque kernel's supports sup_d) and the torque terms
+// of the position gradient, added to grad:
+//   grad[p][k] += 1/2 sum eps_abc (mu[p][a] J_kb u_c - t[p][c] J_kb v_a).
+template <class T>
+void launch_stencil_d2(const StencilArgs& A, const int4* sup_d, void* rec, int rs, hipStream_t s);
+template <class T, class Tio>
+void launch_interp_grad_t(const InterpArgs& A, const int4* sup, const void* rec, int rs, const KernelDesc& K,
+                          const void* gu, const void* gv, const void* mu, const void* t, void* grad, hipStream_t s);
+
 // Fourier solve of the triply periodic Stokes problem on the r2c spectrum
 // [c][kz][ky][kx] (kx in [0, Nx/2]), scaled by 1/(Nx Ny Nz) for the c2r
 template <class T>

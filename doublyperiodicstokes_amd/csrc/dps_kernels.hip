@@ -686,6 +686,189 @@ __global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* _
   }
 }
 
+// Torque-kernel records of the position adjoint (no counterpart in the
+// reference).  Per axis, over the support of the torque kernel: the value phi,
+// the forward derivative phi d (at clamped offsets too, as the forward
+// weights), the position derivative (zero where the offset was clamped) and
+// the second derivative phi (d^2 + d') (zero where clamped), with
+// d = -beta o / (alpha^2 s), d' = -beta / (alpha^2 s^3), s = sqrt(1 - o^2 /
+// alpha^2).  Layout [w x][d x][dpos x][dd x][same for y][z value, ghost
+// class A][z dpos, A][z d, class B][z dd, B] (z with the interpolation
+// weights hx hy w_z, on the interior rows sup.z .. sup.z + sup.w - 1 of the
+// torque kernel's own records).
+template <class T>
+__global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* __restrict__ rec_out, int rs)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.N) return;
+  const KernelDesc& K = A.kd;
+  const double x = A.pos[3 * p], y = A.pos[3 * p + 1], z = A.pos[3 * p + 2];
+  const int4 sup = sup_d[p];
+  const double a2 = K.alpha * K.alpha, bw = K.betaw;
+  T* rec = rec_out + (size_t)p * rs;
+  auto axis = [&](double v, double scale, double* w, double* df, double* dp, double* dd) {
+    const double vc = clamp_off(v, K.thresh);
+    const double ph = es_phi(vc, bw, a2) * scale;
+    const double s = sqrt(1 - vc * vc / a2);
+    const double d = -bw * vc / (a2 * s);
+    const bool cl = fabs(vc) >= K.thresh;
+    *w = ph;
+    *df = ph * d;
+    *dp = cl ? 0.0 : ph * d;
+    *dd = cl ? 0.0 : ph * (d * d - bw / (a2 * s * s * s));
+  };
+  int o = 0;
+  {
+    const int wx = K.wx;
+    for (int i = 0; i < wx; ++i) {
+      double w, df, dp, dd;
+      axis((double)(sup.x + i) * A.hx - x, K.inv_norm3, &w, &df, &dp, &dd);
+      rec[o + i] = (T)w;
+      rec[o + wx + i] = (T)df;
+      rec[o + 2 * wx + i] = (T)dp;
+      rec[o + 3 * wx + i] = (T)dd;
+    }
+    o += 4 * wx;
+    const int wy = K.wy;
+    for (int j = 0; j < wy; ++j) {
+      double w, df, dp, dd;
+      axis((double)(sup.y + j) * A.hy - y, 1.0, &w, &df, &dp, &dd);
+      rec[o + j] = (T)w;
+      rec[o + wy + j] = (T)df;
+      rec[o + 2 * wy + j] = (T)dp;
+      rec[o + 3 * wy + j] = (T)dd;
+    }
+    o += 4 * wy;
+  }
+  const int W = K.wzmax;
+  for (int i = 0; i < 4 * W; ++i) rec[o + i] = (T)0;
+  const double hxhy = A.hx * A.hy;
+  if (A.periodic_z) {
+    for (int q = 0; q < sup.w; ++q) {
+      double w, df, dp, dd;
+      axis((double)(sup.z + q) * A.hz - z, 1.0, &w, &df, &dp, &dd);
+      const double qw = hxhy * A.hz;
+      rec[o + q] = (T)(w * qw);
+      rec[o + W + q] = (T)(dp * qw);
+      rec[o + 2 * W + q] = (T)(df * qw);
+      rec[o + 3 * W + q] = (T)(dd * qw);
+    }
+    return;
+  }
+  // doubly periodic: extended nodes folded onto the interior rows with the
+  // value kernel's factors (class A) or the z-derivative's (class B)
+  const Stencil st = stencil_support(x, y, z, A.hx, A.hy, K, A.zext_d);
+  for (int q = 0; q < st.nz; ++q) {
+    const int e = st.e0 + q;
+    const double sa = A.fA_d[e], sb = A.fB_d[e];
+    if (sa == 0.0 && sb == 0.0) continue;
+    const int r = A.kt_d[e] - sup.z;
+    if (r < 0 || r >= sup.w) continue;   // cannot happen: sup spans every row with a factor
+    double w, df, dp, dd;
+    axis(A.zext_d[e] - z, 1.0, &w, &df, &dp, &dd);
+    const double qw = hxhy * A.zw_d[e];
+    rec[o + r] += (T)(w * qw * sa);
+    rec[o + W + r] += (T)(dp * qw * sa);
+    rec[o + 2 * W + r] += (T)(df * qw * sb);
+    rec[o + 3 * W + r] += (T)(dd * qw * sb);
+  }
+}
+
+// Torque terms of the position gradient: with J_kb the interpolation with the
+// position derivative along k of the forward weight phi d_b,
+//   grad[p][k] += 1/2 sum eps_abc (mu[p][a] J_kb u_c - t[p][c] J_kb v_a).
+template <class T, class Tio>
+__global__ void __launch_bounds__(256) k_interp_grad_t(InterpArgs A, const int4* __restrict__ sup_g,
+                                                       const T* __restrict__ recs, int rs, KernelDesc K,
+                                                       const T* __restrict__ U, const T* __restrict__ V,
+                                                       const Tio* __restrict__ mu, const Tio* __restrict__ tq,
+                                                       Tio* __restrict__ grad)
+{
+  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= A.N) return;
+  const int p = A.order[g];
+  const size_t cs = (size_t)A.Nz * A.Ny * A.Nx;
+  const size_t zs = (size_t)A.Ny * A.Nx;
+  const int4 sup = sup_g[p];
+  const T* rec = recs + (size_t)p * rs;
+  const int W = K.wzmax, nz = sup.w, wx = K.wx, wy = K.wy;
+  const T* xw = rec;
+  const T* xf = xw + wx;
+  const T* xp = xf + wx;
+  const T* xdd = xp + wx;
+  const T* yw = xdd + wx;
+  const T* yf = yw + wy;
+  const T* yp = yf + wy;
+  const T* ydd = yp + wy;
+  const T* zA = ydd + wy;
+  const T* zpA = zA + W;
+  const T* zfB = zpA + W;
+  const T* zddB = zfB + W;
+  const int ncol = wx * wy;
+  const int npts = ncol * nz;
+  const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
+  // J[k][b] applied to component c of u (c != b) and of v (a != b): acc[k][b][i]
+  T au[27], av[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) au[i] = av[i] = 0;
+  for (int t = lane; t < npts; t += 64) {
+    int col, a;
+    const int q = fdiv(t, ncol, incol, &col);
+    const int b = fdiv(col, wx, iwx, &a);
+    int x = sup.x + a, y = sup.y + b;
+    x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+    y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+    int zr_ = sup.z + q;
+    if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+    const size_t idx = (size_t)zr_ * zs + (size_t)y * A.Nx + x;
+    T J[9];
+    J[0] = xdd[a] * yw[b] * zA[q];    // k x, b x
+    J[1] = xp[a] * yf[b] * zA[q];     // k x, b y
+    J[2] = xp[a] * yw[b] * zfB[q];    // k x, b z
+    J[3] = xf[a] * yp[b] * zA[q];     // k y, b x
+    J[4] = xw[a] * ydd[b] * zA[q];    // k y, b y
+    J[5] = xw[a] * yp[b] * zfB[q];    // k y, b z
+    J[6] = xf[a] * yw[b] * zpA[q];    // k z, b x
+    J[7] = xw[a] * yf[b] * zpA[q];    // k z, b y
+    J[8] = xw[a] * yw[b] * zddB[q];   // k z, b z
+    const T u[3] = {U[idx], U[idx + cs], U[idx + 2 * cs]};
+    const T v[3] = {V[idx], V[idx + cs], V[idx + 2 * cs]};
+#pragma unroll
+    for (int kb = 0; kb < 9; ++kb)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        au[kb * 3 + c] += J[kb] * u[c];
+        av[kb * 3 + c] += J[kb] * v[c];
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    au[i] = wave_sum(au[i]);
+    av[i] = wave_sum(av[i]);
+  }
+  if (lane == 0) {
+    // a NULL torque or angular cotangent counts as zero
+    double m[3] = {0, 0, 0}, tt[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c) {
+      if (mu) m[c] = (double)mu[3 * p + c];
+      if (tq) tt[c] = (double)tq[3 * p + c];
+    }
+    // (a, b, c, sign) of the Levi-Civita symbol
+    const int E[6][4] = {{0, 1, 2, 1}, {1, 2, 0, 1}, {2, 0, 1, 1}, {0, 2, 1, -1}, {1, 0, 2, -1}, {2, 1, 0, -1}};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double acc = 0;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) {
+        const int ea = E[e][0], eb = E[e][1], ec = E[e][2];
+        acc += 0.5 * E[e][3] * (m[ea] * (double)au[(k * 3 + eb) * 3 + ec] - tt[ec] * (double)av[(k * 3 + eb) * 3 + ea]);
+      }
+      grad[3 * p + k] = (Tio)((double)grad[3 * p + k] + acc);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Chebyshev transform in z (folded even/odd node pairs)
 // Each workgroup: 64 complex columns (modes) of one component, all nodes.
@@ -1303,6 +1486,24 @@ void launch_interp_grad(const InterpArgs& A, const int4* sup, const void* rec, i
                      (const Tio*)f, (Tio*)grad);
 }
 
+template <class T>
+void launch_stencil_d2(const StencilArgs& A, const int4* sup_d, void* rec, int rs, hipStream_t s)
+{
+  if (A.N == 0) return;
+  hipLaunchKernelGGL((k_stencil_d2<T>), dim3((A.N + 127) / 128), dim3(128), 0, s, A, sup_d, (T*)rec, rs);
+}
+
+template <class T, class Tio>
+void launch_interp_grad_t(const InterpArgs& A, const int4* sup, const void* rec, int rs, const KernelDesc& K,
+                          const void* gu, const void* gv, const void* mu, const void* t, void* grad, hipStream_t s)
+{
+  if (A.N == 0) return;
+  const int waves_per_block = 256 / 64;
+  hipLaunchKernelGGL((k_interp_grad_t<T, Tio>), dim3((A.N + waves_per_block - 1) / waves_per_block), dim3(256),
+                     0, s, A, sup, (const T*)rec, rs, K, (const T*)gu, (const T*)gv, (const Tio*)mu,
+                     (const Tio*)t, (Tio*)grad);
+}
+
 template <class T, class Tout>
 void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s)
 {
@@ -1384,6 +1585,13 @@ void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipS
   template void launch_interp_grad<T, double>(const InterpArgs&, const int4*, const void*, int,      \
                                               const KernelDesc&, const void*, const void*,            \
                                               const void*, const void*, void*, hipStream_t);          \
+  template void launch_stencil_d2<T>(const StencilArgs&, const int4*, void*, int, hipStream_t);     \
+  template void launch_interp_grad_t<T, float>(const InterpArgs&, const int4*, const void*, int,     \
+                                               const KernelDesc&, const void*, const void*,           \
+                                               const void*, const void*, void*, hipStream_t);         \
+  template void launch_interp_grad_t<T, double>(const InterpArgs&, const int4*, const void*, int,    \
+                                                const KernelDesc&, const void*, const void*,          \
+                                                const void*, const void*, void*, hipStream_t);        \
   template void launch_zfwd<T>(const ZArgs&, const void*, void*, hipStream_t);              \
   template void launch_zinv<T>(const ZArgs&, const void*, void*, hipStream_t);              \
   template void launch_bvp<T>(const BvpArgs&, const void*, void*, hipStream_t);             \

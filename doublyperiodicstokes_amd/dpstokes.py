@@ -238,8 +238,9 @@ class DPStokes:
     def mdot_adjoint_device(self, forces, grad_velocities, torques=None, grad_angular=None, positions=True):
         """Adjoint of Mdot at the current positions (dps_mdot_adjoint): returns
         (dL/dF, dL/dT or None, dL/dx or None) for L = <gU, U> + <gW, W>.
-        dL/dT comes with grad_angular; position gradients need forces only
-        (no torques, no grad_angular)."""
+        dL/dT comes when torques or grad_angular are given; position
+        gradients with torques need the derivative torque kernels (not
+        useRegKernel)."""
         self._need()
         torch = _torch()
         gU = self._to_device(grad_velocities, "grad_velocities", 3 * self.N)
@@ -250,8 +251,10 @@ class DPStokes:
             T = self._to_device(torques, "torques", 3 * self.N).to(dt)
         if grad_angular is not None and self.has_torque:
             gW = self._to_device(grad_angular, "grad_angular", 3 * self.N).to(dt)
+        if not positions and T is not None and gW is None:
+            gW = torch.zeros(3 * self.N, dtype=dt, device=self.device)
         gF = torch.empty(3 * self.N, dtype=dt, device=self.device)
-        gT = torch.empty(3 * self.N, dtype=dt, device=self.device) if gW is not None else None
+        gT = torch.empty(3 * self.N, dtype=dt, device=self.device) if (gW is not None or T is not None) else None
         gX = torch.empty(3 * self.N, dtype=dt, device=self.device) if positions else None
         vp = ctypes.c_void_p
         ptr = lambda t: vp(t.data_ptr()) if t is not None else None

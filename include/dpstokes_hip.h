@@ -133,13 +133,16 @@ int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* veloc
        periodic; doubly periodic, up to the discretisation's asymmetry);
        grad_torques needs grad_forces and grad_angular (zeros when W does not
        enter L);
-     grad_positions = dL/dx (3N), forces only: torques, grad_angular and
-       grad_torques must be NULL.  It is
-       -sum_i gU[p,i] D_j u_i(x_p) - sum_i F[p,i] D_j v_i(x_p),
-       with u the velocity grid of the forces, v that of grad_velocities and
-       D_j interpolation with the j-th derivative of the force kernel (zero
-       where an offset is clamped); two spreads, two solves and one gradient
-       interpolation.
+     grad_positions = dL/dx (3N):
+       -sum_i gU[p,i] D_k u_i(x_p) - sum_i F[p,i] D_k v_i(x_p)
+       + 1/2 sum eps_abc (gW[p,a] J_kb u_c(x_p) - T[p,c] J_kb v_a(x_p)),
+       with u the velocity grid of (F, T), v that of (gU, gW), D_k
+       interpolation with the k-th derivative of the force kernel and J_kb
+       with the (k, b) second derivative of the torque kernel, as functions
+       of the particle position (a clamped offset contributes no derivative
+       along its axis); two spreads, two solves and one or two gradient
+       interpolations.  With reg_kernel = 1 torque terms are DPS_ERR_ARG.
+       With grad_positions, grad_torques (dL/dT) needs no grad_angular.
    Outputs may be NULL (not computed).  All arrays use `dtype`. */
 int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, const void* grad_velocities,
                      const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,

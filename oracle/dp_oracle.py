@@ -84,6 +84,7 @@ def lib():
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                  ip, ip, ip, ip, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp]
+        L.orc_interp_pos.argtypes = [ctypes.c_int, ctypes.c_double] + L.orc_interp.argtypes[1:]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -729,13 +730,30 @@ class OracleDPStokes:
         om[2::3] = -1 / 2 * (dxv[1::3] - dyv[0::3])
         return v, om
 
+    def _interp_pos(self, kind, loc, ks, fG, BCs):
+        L = lib()
+        uP = np.zeros(3 * self.N)
+        L.orc_interp_pos(kind, ks.thresh, self.N, _p(np.ascontiguousarray(fG)), ks.betaw, ks.alpha, ks.norm,
+                         loc['wx'], loc['wy'], _p(loc['xclose']), _p(loc['yclose']),
+                         _p(loc['indl']), _p(loc['indr']), _p(loc['xunwrap']), _p(loc['yunwrap']),
+                         _p(loc['zunwrap']), _p(loc['pt_wts']), loc['zstride'], self.Nx, self.Ny,
+                         self.Nz, loc['Nzeff'], _p(BCs), _p(self._ext_buffer(loc)), _p(uP))
+        return uP
+
     def interp_force_kernel(self, kind, uG):
         """Interpolate uG with the force kernel (kind 0) or its x/y/z
         derivative (kind 1/2/3) as a function of the particle position (zero
         where an offset was clamped), folds and ghosts as for the force
         kernel."""
-        return self._interp(kind + 3 if kind else 0, self._loc, self.mono, uG, self.BCs,
-                            self._ext_buffer(self._loc))
+        if kind == 0:
+            return self._interp(0, self._loc, self.mono, uG, self.BCs, self._ext_buffer(self._loc))
+        return self._interp_pos(kind, self._loc, self.mono, uG, self.BCs)
+
+    def interp_torque_kernel_d2(self, k, b, uG):
+        """Interpolate uG with d2/(do_k do_b) of the torque kernel as a
+        function of the particle position (k, b in 0..2), ghosts of the class
+        the first derivative along b uses (BCst_z for z, BCs otherwise)."""
+        return self._interp_pos(10 + 3 * k + b, self._tloc, self.dip, uG, self.BCst_z if b == 2 else self.BCs)
 
     def mdot(self, forces, torques=None):
         """Linear and angular velocities (FCM.py:334-441)."""
@@ -751,24 +769,38 @@ class OracleDPStokes:
         return self.interpolate(uG, tp, self.utG)
 
 
+_EPS = ((0, 1, 2, 1.0), (1, 2, 0, 1.0), (2, 0, 1, 1.0), (0, 2, 1, -1.0), (1, 0, 2, -1.0), (2, 1, 0, -1.0))
+
+
 def adjoint(o, forces, lam_u, lam_w=None, torques=None):
     """Adjoint of Mdot for either oracle (no counterpart in the reference).
     With L = <lam_u, U> + <lam_w, W> and (U, W) = Mdot(F, T) at the current
     positions:
       dL/dF, dL/dT = Mdot(lam_u, lam_w)   (the solve taken as self-adjoint)
-      dL/dx_p = -sum_i lam_u[p,i] D_j u_i(x_p) - sum_i F[p,i] D_j v_i(x_p)
-    (forces only; None with torques), where u is the velocity grid of F, v
-    the velocity grid of lam_u, and D_j interpolation with the j-th
-    derivative of the force kernel: a weight phi(x_g - x_p) changes with x_p
-    as minus the kernel derivative, and ghost folds and copies keep the force
-    kernel's factors because the ghost nodes do not move."""
+      dL/dx_p[k] = - sum_i lam_u[p,i] I_k u_i - sum_i F[p,i] I_k v_i
+                   + 1/2 sum eps_abc lam_w[p,a] J_kb u_c
+                   - 1/2 sum eps_abc T[p,c] J_kb v_a
+    where u is the velocity grid of (F, T), v that of (lam_u, lam_w), I_k
+    interpolation with the k-th derivative of the force kernel and J_kb with
+    the (k, b) second derivative of the torque kernel, all as functions of
+    the particle position: a weight phi(x_g - x_p) changes with x_p as minus
+    the kernel derivative, ghost folds and copies keep the factors of the
+    forward weights (ghost nodes do not move) and a coordinate whose offset
+    was clamped contributes no derivative.  The torque terms follow from
+    W = -1/2 eps_abc D_b u_c (derivative-kernel interpolation, FCM.py:411-426)
+    and the torque forcing 1/2 eps_abc D_b T_c (SpreadInterpWrapper.cpp:66-107).
+    Position gradients are None with the regular torque kernel."""
     forces = np.asarray(forces, dtype=np.float64)
     lam_u = np.asarray(lam_u, dtype=np.float64)
-    gF, gT = o.mdot(lam_u, lam_w)
-    if (torques is not None and np.size(torques) > 0) or (lam_w is not None and np.size(lam_w) > 0):
+    tq = torques is not None and np.size(torques) > 0
+    lw = lam_w is not None and np.size(lam_w) > 0
+    gF, gT = o.mdot(lam_u, lam_w if lw else None)
+    if (tq or lw) and getattr(o, 'reg_kernel', False):
         return gF, gT, None
-    uG = o.solve(o.spread_forces(forces, None))
-    vG = o.solve(o.spread_forces(lam_u, None))
+    T = np.asarray(torques, dtype=np.float64) if tq else None
+    mu = np.asarray(lam_w, dtype=np.float64) if lw else None
+    uG = o.solve(o.spread_forces(forces, T))
+    vG = o.solve(o.spread_forces(lam_u, mu))
     N = o.N
     lam = lam_u.reshape(N, 3)
     F3 = forces.reshape(N, 3)
@@ -777,6 +809,14 @@ def adjoint(o, forces, lam_u, lam_w=None, torques=None):
         du = o.interp_force_kernel(j, uG).reshape(N, 3)
         dv = o.interp_force_kernel(j, vG).reshape(N, 3)
         gx[:, j - 1] = -(lam * du).sum(1) - (F3 * dv).sum(1)
+    if tq or lw:
+        T3 = T.reshape(N, 3) if tq else np.zeros((N, 3))
+        M3 = mu.reshape(N, 3) if lw else np.zeros((N, 3))
+        for k in range(3):
+            for a, b, c, sgn in _EPS:
+                Ju = o.interp_torque_kernel_d2(k, b, uG).reshape(N, 3)
+                Jv = o.interp_torque_kernel_d2(k, b, vG).reshape(N, 3)
+                gx[:, k] += 0.5 * sgn * (M3[:, a] * Ju[:, c] - T3[:, c] * Jv[:, a])
     return gF, gT, gx.ravel()
 
 

@@ -193,20 +193,44 @@ int orc_locate(int nP, const double* xP, double hx, double hy, double Lx, double
   return 0;
 }
 
-/* ES kernel (kind 0) or its x/y/z derivative (kind 1/2/3); include/ESKernels.h:8-46.
-   Kinds 4/5/6 (no counterpart in the reference; the adjoint with respect to
-   positions): the x/y/z derivative of the weight as a function of the
-   particle position, which is zero where the offset was clamped (the weight
-   is then the constant phi(thresh)); the caller passes thresh as alpha. */
+/* ES kernel (kind 0) or its x/y/z derivative (kind 1/2/3); include/ESKernels.h:8-46 */
 static inline double kern3(int kind, const double x[3], double beta, double alpha)
 {
   const double alpha2 = alpha * alpha;
-  if (kind >= 4 && fabs(x[kind - 4]) >= alpha) return 0.0;
   const double e = exp(beta * (sqrt(1 - x[0] * x[0] / alpha2) + sqrt(1 - x[1] * x[1] / alpha2) +
                                sqrt(1 - x[2] * x[2] / alpha2) - 3));
   if (kind == 0) return e;
-  const int c = (kind - 1) % 3;
+  const int c = kind - 1;
   return -beta * x[c] * e / (alpha2 * sqrt(1 - x[c] * x[c] / (alpha2)));
+}
+
+/* Weights as functions of the particle position (no counterpart in the
+   reference; the adjoint with respect to positions).  kind 1..3: first
+   derivative d/do_k of the kernel; kind 10 + 3k + b: second derivative
+   d2/(do_k do_b), the derivative along k of the forward weight
+   phi d_b.  A coordinate whose offset was clamped to thresh is constant: a
+   derivative along k is zero when o_k was clamped, while a clamped o_b keeps
+   its value d_b(thresh) in the weight.  With
+   s_i = sqrt(1 - o_i^2/alpha^2) and d_i = -beta o_i / (alpha^2 s_i):
+   d/do_k = phi d_k, d2/(do_k do_b) = phi (d_k d_b + [k == b] d'_k),
+   d'_k = -beta / (alpha^2 s_k^3). */
+static inline double kern3_pos(int kind, const double x[3], double beta, double alpha, double thresh)
+{
+  int k, b;
+  if (kind >= 10) { k = (kind - 10) / 3; b = (kind - 10) % 3; }
+  else { k = kind - 1; b = -1; }
+  if (fabs(x[k]) >= thresh) return 0.0;
+  const double alpha2 = alpha * alpha;
+  double sq[3], d[3];
+  for (int i = 0; i < 3; ++i) {
+    sq[i] = sqrt(1 - x[i] * x[i] / alpha2);
+    d[i] = -beta * x[i] / (alpha2 * sq[i]);
+  }
+  const double e = exp(beta * (sq[0] + sq[1] + sq[2] - 3));
+  if (b < 0) return e * d[k];
+  double r = d[k] * d[b];
+  if (k == b) r += -beta / (alpha2 * sq[k] * sq[k] * sq[k]);
+  return e * r;
 }
 
 /* fold of the extended grid into the interior grid; include/BoundaryConditions.h:22-361
@@ -416,11 +440,38 @@ void orc_spread(int kind, int nP, const double* fP, double betaw, double alpha, 
  * extended grid: its ghost region at a 'none' end is read as-is (the reference
  * reads whatever the last spread left there); pass a zeroed buffer to read 0.
  */
+static void interp_impl(int kind, int posmode, double thresh, int nP, const double* fG, double betaw, double alpha,
+                        double norm, int wx, int wy, const int* xclose, const int* yclose, const int* indl,
+                        const int* indr, const double* xunwrap, const double* yunwrap,
+                        const double* zunwrap, const double* pt_wts, int zstride, int Nx, int Ny,
+                        int Nz, int Nzeff, const int* BCs, double* Fe_io, double* uP);
+
 void orc_interp(int kind, int nP, const double* fG, double betaw, double alpha, double norm,
                 int wx, int wy, const int* xclose, const int* yclose, const int* indl,
                 const int* indr, const double* xunwrap, const double* yunwrap,
                 const double* zunwrap, const double* pt_wts, int zstride, int Nx, int Ny,
                 int Nz, int Nzeff, const int* BCs, double* Fe_io, double* uP)
+{
+  interp_impl(kind, 0, 0.0, nP, fG, betaw, alpha, norm, wx, wy, xclose, yclose, indl, indr, xunwrap, yunwrap,
+              zunwrap, pt_wts, zstride, Nx, Ny, Nz, Nzeff, BCs, Fe_io, uP);
+}
+
+/* Interpolation with position-derivative weights (kern3_pos kinds). */
+void orc_interp_pos(int kind, double thresh, int nP, const double* fG, double betaw, double alpha, double norm,
+                    int wx, int wy, const int* xclose, const int* yclose, const int* indl,
+                    const int* indr, const double* xunwrap, const double* yunwrap,
+                    const double* zunwrap, const double* pt_wts, int zstride, int Nx, int Ny,
+                    int Nz, int Nzeff, const int* BCs, double* Fe_io, double* uP)
+{
+  interp_impl(kind, 1, thresh, nP, fG, betaw, alpha, norm, wx, wy, xclose, yclose, indl, indr, xunwrap, yunwrap,
+              zunwrap, pt_wts, zstride, Nx, Ny, Nz, Nzeff, BCs, Fe_io, uP);
+}
+
+static void interp_impl(int kind, int posmode, double thresh, int nP, const double* fG, double betaw, double alpha,
+                        double norm, int wx, int wy, const int* xclose, const int* yclose, const int* indl,
+                        const int* indr, const double* xunwrap, const double* yunwrap,
+                        const double* zunwrap, const double* pt_wts, int zstride, int Nx, int Ny,
+                        int Nz, int Nzeff, const int* BCs, double* Fe_io, double* uP)
 {
   const int Nxeff = Nx + wx + (wx % 2), Nyeff = Ny + wy + (wy % 2);
   const int evenx = -1 * (wx % 2) + 1, eveny = -1 * (wy % 2) + 1;
@@ -438,7 +489,8 @@ void orc_interp(int kind, int nP, const double* fG, double betaw, double alpha, 
           x[0] = xunwrap[i + p * wx];
           x[1] = yunwrap[j + p * wy];
           x[2] = zunwrap[k + p * zstride];
-          const double delta = kern3(kind, x, betaw, alpha) / norm3;
+          const double delta = (posmode ? kern3_pos(kind, x, betaw, alpha, thresh) : kern3(kind, x, betaw, alpha)) /
+                               norm3;
           const unsigned g = at3(ii + i - wx / 2 + evenx, jj + j - wy / 2 + eveny, indl[p] + k,
                                  Nxeff, Nyeff);
           const double wgt = pt_wts[k + p * zstride];

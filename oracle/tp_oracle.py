@@ -207,6 +207,38 @@ class OracleTPStokes:
         U = uG.reshape(-1, 3)
         return np.stack([np.sum(U[g, d] * W, axis=(1, 2, 3)) for d in range(3)], 1).ravel()
 
+    def interp_torque_kernel_d2(self, k, b, uG):
+        """Interpolate uG with d2/(do_k do_b) of the torque kernel as a
+        function of the particle position (zero when o_k was clamped):
+        phi (d_k d_b + [k == b] d'_k), d_i = -beta o_i / (alpha^2 s_i),
+        d'_i = -beta / (alpha^2 s_i^3), s_i = sqrt(1 - o_i^2 / alpha^2)."""
+        ks = self.dip
+        (ix, ox), (iy, oy), (iz, oz) = self._tloc
+        offs = (ox, oy, oz)
+        a2 = ks.alpha * ks.alpha
+        shapes = [(1, 1, 1, -1), (1, 1, -1, 1), (1, -1, 1, 1)]
+        W = _kern3(0, ox, oy, oz, ks.betaw, ks.alpha) / ks.norm ** 3 * (self.hx * self.hy * self.hz)
+
+        def axis(i, v):
+            return v.reshape((v.shape[0],) + shapes[i][1:])
+
+        with np.errstate(invalid="ignore", divide="ignore"):
+            s_k = np.sqrt(1 - offs[k] ** 2 / a2)
+            s_b = np.sqrt(1 - offs[b] ** 2 / a2)
+            d_k = -ks.betaw * offs[k] / (a2 * s_k)
+            d_b = -ks.betaw * offs[b] / (a2 * s_b)
+            if k == b:
+                W = W * axis(k, d_k * d_k - ks.betaw / (a2 * s_k ** 3))
+            else:
+                W = W * axis(k, d_k) * axis(b, d_b)
+            # a clamped o_k is constant (no derivative along k); a clamped
+            # o_b keeps its value d_b(thresh) in the forward weight
+            W = W * axis(k, (np.abs(offs[k]) < ks.thresh).astype(float))
+            W = np.nan_to_num(W, nan=0.0, posinf=0.0, neginf=0.0)
+        g = self._flat_index(self._tloc)
+        U = uG.reshape(-1, 3)
+        return np.stack([np.sum(U[g, d] * W, axis=(1, 2, 3)) for d in range(3)], 1).ravel()
+
     def mdot(self, forces, torques=None):
         forces = np.asarray(forces, dtype=np.float64)
         tp = self.has_torque and torques is not None and np.size(torques) > 0

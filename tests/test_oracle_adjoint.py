@@ -11,7 +11,11 @@ which retypes its formula:
 * doubly periodic: the solve is self-adjoint only up to the discretisation
   (Chebyshev collocation against Clenshaw-Curtis weights), so the gradient
   equals the finite difference to that asymmetry (observed 1e-6 .. 9e-4 on
-  the grids below; bound 2e-3).
+  the grids below; bound 2e-3);
+* with torques, the second-derivative term alone matches the finite
+  difference of the torque kernel's derivative interpolation (1e-6), and
+  the whole position gradient matches FD as above (TP observed 2e-9, DP
+  8e-6 .. 1.1e-4).
 """
 import numpy as np
 import pytest
@@ -100,9 +104,79 @@ def test_doubly_periodic_adjoint_within_solve_asymmetry(geom, nz):
     assert np.linalg.norm(gx - fd) <= 2e-3 * np.linalg.norm(fd)
 
 
-def test_torques_give_no_position_gradient():
+@pytest.mark.parametrize("geom", ["nowall", "bottom", "slit", "periodic"])
+def test_second_derivative_term_matches_finite_differences(geom):
+    """d/dx_p,k of the torque kernel's b-derivative interpolation of a fixed
+    grid equals minus the (k, b) second-derivative interpolation, with the
+    ghost class of b (BCst_z for z, BCs otherwise)."""
+    kw = dict(has_torque=True, w_d=6, betaw_d=1.785 * 6)
+    if geom == "periodic":
+        o = OracleTPStokes(8.0, 8.0, 0.0, 6.3, 16, 16, 16, ETA, 6, 1.785 * 6, **kw)
+        zlo, zhi = 0.0, 6.3
+    else:
+        o = OracleDPStokes(geom, 8.0, 8.0, 0.0, 6.3, 16, 16, 17, ETA, 6, 1.785 * 6, **kw)
+        zlo, zhi = (1.6, 4.7) if geom == "nowall" else (0.3, 4.7 if geom == "bottom" else 6.0)
+    N = 4
+    rng = np.random.RandomState(9)
+    pos = np.stack([rng.uniform(0, 8, N), rng.uniform(0, 8, N), rng.uniform(zlo, zhi, N)], 1).ravel()
+    G = rng.normal(size=(o.Ny, o.Nx, o.Nz, 3))
+
+    def first(b):
+        if geom == "periodic":
+            return o._interp(b + 1, o._tloc, o.dip, G).ravel()
+        return o._interp(b + 1, o._tloc, o.dip, G, o.BCst_z if b == 2 else o.BCs, o._ext_buffer(o._tloc))
+
+    h = 1e-6
+    for b in range(3):
+        for k in range(3):
+            o.set_positions(pos)
+            got = -o.interp_torque_kernel_d2(k, b, G)
+            fd = np.zeros(3 * N)
+            for p in range(N):
+                pp = pos.copy()
+                pp[3 * p + k] += h
+                o.set_positions(pp)
+                up = first(b)[3 * p:3 * p + 3]
+                pp[3 * p + k] -= 2 * h
+                o.set_positions(pp)
+                um = first(b)[3 * p:3 * p + 3]
+                fd[3 * p:3 * p + 3] = (up - um) / (2 * h)
+            assert np.linalg.norm(got - fd) <= 1e-6 * np.linalg.norm(fd), (k, b)
+
+
+@pytest.mark.parametrize("geom,nz,tol", [("periodic", 16, 1e-6), ("bottom", 49, 2e-3), ("slit", 33, 2e-3),
+                                         ("nowall", 49, 2e-3)])
+def test_adjoint_with_torques_matches_finite_differences(geom, nz, tol):
+    kw = dict(has_torque=True, w_d=6, betaw_d=1.785 * 6)
+    if geom == "periodic":
+        o = OracleTPStokes(8.0, 8.0, 0.0, 8.0, 16, 16, 16, ETA, 6, 1.785 * 6, **kw)
+        L, zlo, zhi, N = 8.0, 0.0, 8.0, 5
+    else:
+        o = OracleDPStokes(geom, 16.0, 16.0, 0.0, 10.0, 32, 32, nz, ETA, 6, 1.785 * 6, **kw)
+        L, N = 16.0, 4
+        zlo, zhi = (1.6, 8.4) if geom == "nowall" else (0.5, 8.4 if geom == "bottom" else 9.5)
+    rng = np.random.RandomState(2)
+    pos = np.stack([rng.uniform(0, L, N), rng.uniform(0, L, N), rng.uniform(zlo, zhi, N)], 1).ravel()
+    F, T, lu, lw = (rng.normal(size=3 * N) for _ in range(4))
+    o.set_positions(pos)
+    gF, gT, gx = adjoint(o, F, lu, lw, T)
+    h = 1e-5
+    fd = np.zeros(3 * N)
+    for k in range(3 * N):
+        vals = []
+        for sgn in (1, -1):
+            p = pos.copy()
+            p[k] += sgn * h
+            o.set_positions(p)
+            u, w = o.mdot(F, T)
+            vals.append(lu @ u + lw @ w)
+        fd[k] = (vals[0] - vals[1]) / (2 * h)
+    assert np.linalg.norm(gx - fd) <= tol * np.linalg.norm(fd)
+
+
+def test_regular_torque_kernel_gives_no_position_gradient():
     o = OracleDPStokes("slit", 8.0, 8.0, 0.0, 6.3, 16, 16, 17, ETA, 6, 1.785 * 6, has_torque=True, w_d=6,
-                       betaw_d=1.785 * 6)
+                       betaw_d=1.785 * 6, reg_kernel=True)
     rng = np.random.RandomState(3)
     o.set_positions(np.stack([rng.uniform(0, 8, 3), rng.uniform(0, 8, 3), rng.uniform(0.5, 5.8, 3)], 1).ravel())
     F, T, lu, lw = (rng.normal(size=9) for _ in range(4))
