@@ -34,6 +34,10 @@ WORKLOADS = {
     # name: (geometry, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque)
     "slit16k": ("slit", 16384, 128.0, 128.0, 32.0, 1.5, 30.5, 256, 256, 96, True),
     "bottom2k": ("bottom", 2048, 64.0, 64.0, 24.0, 1.5, 20.0, 128, 128, 64, False),
+    # differentiation workload: the top of the z grid is 24 + 1.5 w h / 2
+    "bottom8k": ("bottom", 8192, 96.0, 96.0, 26.25, 1.5, 24.0, 192, 192, 80, True),
+    # sedimented monolayer: z = 1.1 + |N(0, 0.3)|, h = 1 (alpha = 3), Lz = 8
+    "monolayer256k": ("bottom", 262144, 1024.0, 1024.0, 8.0, None, None, 1024, 1024, 48, True),
 }
 
 
@@ -42,8 +46,14 @@ def make_inputs(wl, seed=0):
     rng = np.random.RandomState(seed)
     x = rng.uniform(0, Lx, N)
     y = rng.uniform(0, Ly, N)
-    z = rng.uniform(zlo, zhi, N)
-    F = rng.normal(0, 1, 3 * N)
+    if zlo is None:
+        # monolayer: z = 1.1 + |N(0, 0.3)|; forces (N(0,.1), N(0,.1), -1 + N(0,.1))
+        z = 1.1 + np.abs(rng.normal(0, 0.3, N))
+        F = rng.normal(0, 0.1, 3 * N)
+        F[2::3] -= 1.0
+    else:
+        z = rng.uniform(zlo, zhi, N)
+        F = rng.normal(0, 1, 3 * N)
     T = rng.normal(0, 1, 3 * N) if torque else None
     return np.stack([x, y, z], 1).ravel(), F, T
 
@@ -222,6 +232,59 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args, wl):
+    """Forward Mdot and its adjoint (gradients with respect to positions,
+    forces and torques of L = <g, U> + <h, W>, g, h iid N(0, 1)) on one
+    workload, mixed precision, with the FP64 path's gradients for comparison."""
+    import torch
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    pos, F, T = make_inputs(wl)
+    rng = np.random.RandomState(1)
+    gU = rng.normal(size=3 * N)
+    gW = rng.normal(size=3 * N) if torque else None
+    dev = torch.device("cuda", 0)
+    res = {}
+    for prec, dt in (("mixed", torch.float32), ("double", torch.float64)):
+        s = gpu_solver(wl, prec)
+        s.setPositions(pos)
+        t = lambda a: torch.from_numpy(a).to(dev, dt) if a is not None else None
+        Fd, Td, gUd, gWd = t(F), t(T), t(gU), t(gW)
+        steps = args.steps if prec == "mixed" else max(3, args.steps // 5)
+        ms_f = time_steps(torch, lambda: s.mdot_device(Fd, Td), steps, args.warmup)
+        ms_a = time_steps(torch, lambda: s.mdot_adjoint_device(Fd, gUd, torques=Td, grad_angular=gWd),
+                          steps, max(1, args.warmup // 2))
+        gF, gT, gX = s.mdot_adjoint_device(Fd, gUd, torques=Td, grad_angular=gWd)
+        torch.cuda.synchronize()
+        res[prec] = (ms_f, ms_a, gX.double().cpu().numpy(), gF.double().cpu().numpy())
+        del s
+    ms_f, ms_a = res["mixed"][0], res["mixed"][1]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {
+        "metric": "adjoint Mdot/s (forward + gradients w.r.t. positions, forces%s)" % (" and torques" if torque else ""),
+        "value": 1000.0 / (ms_f + ms_a),
+        "unit": "forward+adjoint/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_f + ms_a,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grids + f64 BVP",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s"
+                   % (args.workload, geom, N, Lx, Ly, zmax, nx, ny, nz, "F+T" if torque else "F"),
+                   "precision": "mixed"},
+        "forward_ms": ms_f,
+        "adjoint_ms": ms_a,
+        "fp64_forward_ms": res["double"][0],
+        "fp64_adjoint_ms": res["double"][1],
+        "rel_l2_dLdx_mixed_vs_fp64": rel(res["mixed"][2], res["double"][2]),
+        "rel_l2_dLdF_mixed_vs_fp64": rel(res["mixed"][3], res["double"][3]),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +300,8 @@ def main():
                     help="also run a plain copy of this many MiB (profiler byte-counter calibration)")
     ap.add_argument("--batch", action="store_true",
                     help="the 64-instance batched workload instead of the single slit instance")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the forward Mdot and its adjoint (position, force and torque gradients)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -244,6 +309,9 @@ def main():
         sys.exit(2)
     if args.batch:
         bench_batch(args)
+        return
+    if args.adjoint:
+        bench_adjoint(args, WORKLOADS[args.workload])
         return
     import torch
     wl = WORKLOADS[args.workload]
