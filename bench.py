@@ -80,12 +80,13 @@ def stage_bytes(wl, s, reg=False):
     }
 
 
-def gpu_solver(wl, precision, reg=False):
+def gpu_solver(wl, precision, reg=False, solver="auto"):
     from doublyperiodicstokes_amd import DPStokes, StokesParameters
     geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
     par = StokesParameters(viscosity=1 / (6 * np.pi), Lx=Lx, Ly=Ly, zmin=0.0, zmax=zmax,
                            w=6, w_d=6 if torque else 0, beta=1.785 * 6, beta_d=1.785 * 6,
-                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision, useRegKernel=reg)
+                           nx=nx, ny=ny, nz=nz, mode=geom, precision=precision, useRegKernel=reg,
+                           solver=solver)
     s = DPStokes()
     s.initialize(par, N)
     return s
@@ -300,6 +301,8 @@ def main():
                     help="also run a plain copy of this many MiB (profiler byte-counter calibration)")
     ap.add_argument("--batch", action="store_true",
                     help="the 64-instance batched workload instead of the single slit instance")
+    ap.add_argument("--solver", default="auto", choices=["auto", "lane"],
+                    help="spectral solve: fused lane-group kernel (auto) or MFMA z-GEMMs + one-lane BVP (lane)")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the forward Mdot and its adjoint (position, force and torque gradients)")
     args = ap.parse_args()
@@ -322,7 +325,7 @@ def main():
     Td = torch.from_numpy(T.astype(np.float32)).to(dev) if torque else None
 
     reg = args.reg_kernel and torque
-    mixed = gpu_solver(wl, "mixed", reg)
+    mixed = gpu_solver(wl, "mixed", reg, args.solver)
     mixed.setPositions(pos)
     ms = time_steps(torch, lambda: mixed.mdot_device(Fd, Td), args.steps, args.warmup)
     U32, W32 = mixed.mdot_device(Fd, Td)
@@ -362,9 +365,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 grids + f64 BVP",
         "data": "synthetic",
-        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s%s"
+        "config": {"workload": "%s: %s N=%d L=%gx%g zmax=%g grid %dx%dx%d w=w_d=6 betaw=10.71 %s%s%s"
                    % (args.workload, geom, N, Lx, Ly, zmax, nx, ny, nz, "F+T" if torque else "F",
-                      " regular torque kernel" if reg else ""),
+                      " regular torque kernel" if reg else "",
+                      " lane solver" if args.solver == "lane" else ""),
                    "precision": "mixed"},
         "stage_us": stage_us,
         "stage_GBps": stage_gbs,
@@ -376,7 +380,7 @@ def main():
     }
     del mixed
     if not args.no_fp64:
-        f64 = gpu_solver(wl, "double", reg)
+        f64 = gpu_solver(wl, "double", reg, args.solver)
         f64.setPositions(pos)
         Fd64 = torch.from_numpy(F).to(dev)
         Td64 = torch.from_numpy(T).to(dev) if torque else None

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libdpstokes_hip.so"
-SOURCES = ["dps_api.hip", "dps_kernels.hip", "dps_spectral.hip", "dps_fft.hip"]
+SOURCES = ["dps_api.hip", "dps_kernels.hip", "dps_spectral.hip", "dps_fft.hip", "dps_zgemm.hip"]
 HEADERS = ["dps_core.h", "dps_fft.h", "dps_internal.h", "dps_seg.h", "dps_tables.h"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 

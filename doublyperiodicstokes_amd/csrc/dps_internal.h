@@ -168,6 +168,10 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
 template <class T, class Tin> void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s);
 template <class T, class Tout> void launch_interp(const InterpArgs& A, void* vel, void* ang, hipStream_t s);
 size_t z_lds_bytes(const ZArgs& A, bool mixed);
+// z transforms of the one-lane chain as MFMA GEMMs (false: shape not
+// supported, the caller keeps the VALU kernels)
+template <class T> bool launch_zfwd_mma(const ZArgs& A, const void* in, void* out, hipStream_t s);
+template <class T> bool launch_zinv_mma(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_zinv(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_bvp(const BvpArgs& A, const void* in, void* out, hipStream_t s);

@@ -1521,6 +1521,7 @@ size_t zinv_lds(const ZArgs& A) { return (size_t)A.N * ZCols<T>::value * sizeof(
 template <class T>
 void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s)
 {
+  if (launch_zfwd_mma<T>(A, in, out, s)) return;
   typedef typename C2<T>::t V;
   constexpr int CW = ZCols<T>::value;
   const size_t lds = zfwd_lds<T>(A);
@@ -1532,6 +1533,7 @@ void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s)
 template <class T>
 void launch_zinv(const ZArgs& A, const void* in, void* out, hipStream_t s)
 {
+  if (launch_zinv_mma<T>(A, in, out, s)) return;
   typedef typename C2<T>::t V;
   constexpr int CW = ZCols<T>::value;
   const size_t lds = zinv_lds<T>(A);

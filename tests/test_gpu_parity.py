@@ -98,20 +98,24 @@ def test_mdot_fp64_matches_oracle(case):
     s.clear()
 
 
-@pytest.mark.parametrize("case", CASES[2:5], ids=[c[0] for c in CASES[2:5]])
-def test_lane_solver_fp64_matches_oracle(case):
-    """The one-wavenumber-per-lane spectral path (separate z-transform kernels)."""
-    say("case %s fp64 mdot, lane solver" % case[0])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("case", CASES[2:], ids=[c[0] for c in CASES[2:]])
+def test_lane_solver_matches_oracle(case, precision):
+    """The one-wavenumber-per-lane spectral path: the z transforms as MFMA
+    GEMMs (odd and even Nz, Nz = 20 .. 150), the BVP one mode per lane."""
+    say("case %s %s mdot, lane solver" % (case[0], precision))
     pos, F, T = make_case(case, seed=3)
     ref = oracle_for(case)
     ref.set_positions(pos)
     v_ref, w_ref = ref.mdot(F, T)
-    s = gpu_for(case, "double", solver="lane")
+    s = gpu_for(case, precision, solver="lane")
     s.setPositions(pos)
     v, w = s.Mdot(F, T)
-    ev, ew = rel_l2(v, v_ref), rel_l2(w, w_ref)
+    ev = rel_l2(v, v_ref)
+    ew = rel_l2(w, w_ref) if T is not None else 0.0
     say("  rel-L2 U = %.3e  W = %.3e" % (ev, ew))
-    assert ev <= TOL64 and ew <= TOL64
+    tol = TOL64 if precision == "double" else TOLMIX
+    assert ev <= tol and ew <= tol
     s.clear()
 
 
