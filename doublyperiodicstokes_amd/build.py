@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libdpstokes_hip.so"
 SOURCES = ["dps_api.hip", "dps_kernels.hip", "dps_spectral.hip", "dps_fft.hip", "dps_zgemm.hip"]
-HEADERS = ["dps_core.h", "dps_fft.h", "dps_internal.h", "dps_seg.h", "dps_tables.h"]
+HEADERS = ["dps_core.h", "dps_fft.h", "dps_internal.h", "dps_mma.h", "dps_seg.h", "dps_tables.h"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
