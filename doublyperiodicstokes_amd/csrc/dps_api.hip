@@ -118,6 +118,7 @@ struct dps_handle {
   bool prof = false;
   hipEvent_t ev[9] = {};
   bool have_timings = false;
+  bool have_reg_events = false;   // the profiled Mdot took the regular-kernel path
 
   template <class X>
   int alloc(X** p, size_t count)
@@ -281,9 +282,17 @@ int build(dps_handle* h)
   }
   h->cheb = build_cheb_tables(h->Nz, h->H);
   h->ct = build_cheb_transform(h->Nz);
-  // the regular torque kernel's curls act on Chebyshev coefficients between
-  // the unfused z-transform and BVP kernels
-  h->fused = (P.solver == DPS_SOLVER_AUTO) && !h->reg && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
+  h->fused = (P.solver == DPS_SOLVER_AUTO) && spectral_shape(h->Nz, &h->sp_lpm, &h->sp_e);
+  if (h->fused && h->reg) {
+    // the regular torque kernel's curls act on Chebyshev coefficients: around
+    // the fused solve they need the matrix-core z transforms
+    ZArgs Z{};
+    Z.N = h->Nz;
+    Z.NH = (h->Nz + 1) / 2;
+    Z.NE = (h->Nz + 1) / 2;
+    Z.NO = h->Nz / 2;
+    h->fused = (h->prec == DPS_PREC_MIXED) ? zgemm_supported<float>(Z) : zgemm_supported<double>(Z);
+  }
   if (h->fused) {
     // up to 64 KiB per workgroup keeps several workgroups per CU; beyond
     // that, what the device allows, else the unfused chain
@@ -854,6 +863,48 @@ int do_solve_reg(dps_handle* h, hipStream_t s)
   ZArgs Z0 = Z;
   Z0.nbasis = 0;   // the vorticity's inverse carries no wall correction
   const double nxy = (double)h->Nx * (double)h->Ny;
+  if (h->fused) {
+    // around the fused solve, in values at the nodes: the torque curl's
+    // coefficients transformed back and added to the forcing, the velocity
+    // transformed forward for its curl (the inverse tables divide by Nx Ny,
+    // the curls restore it)
+    ZArgs Za = Z0;
+    Za.accumulate = 1;
+    bool ok;
+    if (mx) {
+      ok = launch_zfwd_mma<float>(Z, h->d_specT, h->d_specC, s);
+      launch_curl_cheb<float>(h->d_specC, h->d_specB, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+      ok = ok && launch_zinv_mma<float>(Za, h->d_specB, h->d_specA, s);
+    } else {
+      ok = launch_zfwd_mma<double>(Z, h->d_specT, h->d_specC, s);
+      launch_curl_cheb<double>(h->d_specC, h->d_specB, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+      ok = ok && launch_zinv_mma<double>(Za, h->d_specB, h->d_specA, s);
+    }
+    if (!ok) return fail(DPS_ERR_STATE, "matrix-core z transform unavailable for this nz");
+    HIPCHK(hipGetLastError());
+    mark(h, 3, s);
+    const SpecArgs SA = spec_args(h);
+    if (mx) launch_spectral<float>(SA, h->d_specA, h->d_specB, s);
+    else launch_spectral<double>(SA, h->d_specA, h->d_specB, s);
+    HIPCHK(hipGetLastError());
+    mark(h, 4, s);
+    if (mx) {
+      ok = launch_zfwd_mma<float>(Z, h->d_specB, h->d_specC, s);
+      launch_curl_cheb<float>(h->d_specC, h->d_specT, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+      ok = ok && launch_zinv_mma<float>(Z0, h->d_specT, h->d_specC, s);
+    } else {
+      ok = launch_zfwd_mma<double>(Z, h->d_specB, h->d_specC, s);
+      launch_curl_cheb<double>(h->d_specC, h->d_specT, h->Nx, h->Ny, h->Nz, h->Nm, h->d_kxs, h->d_kys, h->H, nxy, 0, s);
+      ok = ok && launch_zinv_mma<double>(Z0, h->d_specT, h->d_specC, s);
+    }
+    if (!ok) return fail(DPS_ERR_STATE, "matrix-core z transform unavailable for this nz");
+    HIPCHK(hipGetLastError());
+    mark(h, 5, s);
+    if ((rc = run_fft(h, false, h->d_specB, h->d_grid, s))) return rc;
+    if ((rc = run_fft(h, false, h->d_specC, h->d_gridT, s))) return rc;
+    mark(h, 6, s);
+    return DPS_OK;
+  }
   const BvpArgs B = bvp_args(h);
   if (mx) {
     launch_zfwd<float>(Z, h->d_specA, h->d_specB, s);
@@ -1102,6 +1153,7 @@ int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* veloc
   if ((rc = do_interp(h, velocities, torques ? angular : nullptr, dtype, s))) return rc;
   mark(h, 7, s);
   h->have_timings = h->prof;
+  h->have_reg_events = h->reg && torques;
   return DPS_OK;
 }
 
@@ -1428,7 +1480,9 @@ int dps_get_timings(dps_handle* h, double* out)
   out[0] = 0.0;   // locate/sort happens in dps_set_positions
   for (int i = 0; i < 7; ++i) {
     int a = i, b = i + 1;
-    if (h->fused || h->geometry == DPS_MODE_PERIODIC) {
+    // the regular torque kernel's path records every event (torque curl and
+    // vorticity as the z stages) unless triply periodic
+    if ((h->fused && !(h->reg && h->have_reg_events)) || h->geometry == DPS_MODE_PERIODIC) {
       if (i == 2 || i == 4) {
         out[i + 1] = 0.0;
         continue;

@@ -142,6 +142,7 @@ struct ZArgs {
   const double *kxs, *kys, *zpts;
   double Hs;
   double inv_nxy;   // 1/(Nx Ny): the c2r transform is unnormalised
+  int accumulate;   // matrix-core inverse only: add the values to `out`
 };
 
 struct BvpArgs {
@@ -170,6 +171,7 @@ template <class T, class Tout> void launch_interp(const InterpArgs& A, void* vel
 size_t z_lds_bytes(const ZArgs& A, bool mixed);
 // z transforms of the one-lane chain as MFMA GEMMs (false: shape not
 // supported, the caller keeps the VALU kernels)
+template <class T> bool zgemm_supported(const ZArgs& A);
 template <class T> bool launch_zfwd_mma(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> bool launch_zinv_mma(const ZArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_zfwd(const ZArgs& A, const void* in, void* out, hipStream_t s);

@@ -1302,59 +1302,78 @@ template void launch_tp_solve<double>(const void*, void*, int, int, int, double,
 
 // Half curl of a Fourier-Chebyshev field (the regular-kernel torque path,
 // FCM.py:372-396 for the torque grid, Solvers.py:159-179 for the velocity).
-// One thread per (kx, ky) mode walks the levels from the top coefficient down,
-// carrying the z-derivative recurrence of the x and y components
-// (Chebyshev.py:68-94): D_k = D_{k+2} + (2/H)(k+1) f_{k+1}, D_0 halved.
-// Loads and stores are coalesced across modes.  FP64 arithmetic.
+// The z derivative follows the coefficient recurrence (Chebyshev.py:68-94)
+// D_k = D_{k+2} + (2/H)(k+1) f_{k+1}, D_0 halved, whose even and odd levels
+// form two independent chains.  Four threads per (kx, ky) mode: output x or y
+// (the chain of the other component's derivative) times level parity, each
+// walking its levels from the top down; the x threads also write z, which
+// needs no derivative.  Loads and stores are coalesced across modes.  FP64.
 template <class T>
 __global__ void k_curl_cheb(const typename C2<T>::t* __restrict__ in, typename C2<T>::t* __restrict__ out,
                             int Nx, int Ny, int Nz, int Nm, const double* __restrict__ kxs,
                             const double* __restrict__ kys, double two_over_H, double scale, int add)
 {
   typedef typename C2<T>::t V;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= Nm) return;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = tid % Nm, role = tid / Nm;   // role: (output x | y) x parity
+  if (role >= 4) return;
+  const int oc = role >> 1, par = role & 1;   // oc 0: x (derivative of y), 1: y (derivative of x)
   const int Nkx = Nx / 2 + 1;
   const int iy = m / Nkx, ix = m - iy * Nkx;
   const double dx = (Nx % 2 == 0 && ix == Nx / 2) ? 0.0 : kxs[ix];
   const double dy = (Ny % 2 == 0 && iy == Ny / 2) ? 0.0 : kys[iy];
   const size_t cs = (size_t)Nz * Nm;
   const double h = 0.5 * scale;
-  // recurrence state of the x and y components: D_{k+1}, D_{k+2}, f_{k+1}
-  double x1r = 0, x1i = 0, x2r = 0, x2i = 0, fxr = 0, fxi = 0;
-  double y1r = 0, y1i = 0, y2r = 0, y2i = 0, fyr = 0, fyi = 0;
-  for (int k = Nz - 1; k >= 0; --k) {
+  const V* fsrc = in + (oc == 0 ? cs : 0);   // the differentiated component
+  // levels of this parity from the top: D_k = D_{k+2} + c (k+1) f_{k+1}
+  double Dr = 0, Di = 0;
+  int k = Nz - 1;
+  if ((k & 1) != par) --k;
+#pragma unroll 4
+  for (; k >= 0; k -= 2) {
     const size_t o = (size_t)k * Nm + m;
-    const V a0 = in[o], a1 = in[o + cs], a2 = in[o + 2 * cs];
-    const double c = two_over_H * (double)(k + 1);
-    double zxr = x2r + c * fxr, zxi = x2i + c * fxi;   // d/dz of component x at level k
-    double zyr = y2r + c * fyr, zyi = y2i + c * fyi;
-    x2r = x1r; x2i = x1i; x1r = zxr; x1i = zxi;
-    y2r = y1r; y2i = y1i; y1r = zyr; y1i = zyi;
-    fxr = (double)a0.x; fxi = (double)a0.y;
-    fyr = (double)a1.x; fyi = (double)a1.y;
-    if (k == 0) { zxr *= 0.5; zxi *= 0.5; zyr *= 0.5; zyi *= 0.5; }
-    const double azr = (double)a2.x, azi = (double)a2.y;
-    // (i dy az - dz ay, dz ax - i dx az, i dx ay - i dy ax) / 2
-    double w[6];
-    w[0] = h * (-dy * azi - zyr);
-    w[1] = h * (dy * azr - zyi);
-    w[2] = h * (zxr + dx * azi);
-    w[3] = h * (zxi - dx * azr);
-    w[4] = h * (-dx * fyi + dy * fxi);
-    w[5] = h * (dx * fyr - dy * fxr);
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      V v;
+    if (k + 1 < Nz) {
+      const V f = fsrc[o + Nm];
+      const double c = two_over_H * (double)(k + 1);
+      Dr += c * (double)f.x;
+      Di += c * (double)f.y;
+    }
+    const double zr = (k == 0) ? 0.5 * Dr : Dr, zi = (k == 0) ? 0.5 * Di : Di;
+    const V az = in[o + 2 * cs];
+    double w0, w1;
+    if (oc == 0) {   // (i dy az - dz ay) / 2
+      w0 = h * (-dy * (double)az.y - zr);
+      w1 = h * (dy * (double)az.x - zi);
+    } else {         // (dz ax - i dx az) / 2
+      w0 = h * (zr + dx * (double)az.y);
+      w1 = h * (zi - dx * (double)az.x);
+    }
+    V* d = out + o + (size_t)oc * cs;
+    V v;
+    if (add) {
+      const V b = *d;
+      v.x = (T)((double)b.x + w0);
+      v.y = (T)((double)b.y + w1);
+    } else {
+      v.x = (T)w0;
+      v.y = (T)w1;
+    }
+    *d = v;
+    if (oc == 0) {   // z: (i dx ay - i dy ax) / 2
+      const V ax = in[o], ay = in[o + cs];
+      const double z0 = h * (-dx * (double)ay.y + dy * (double)ax.y);
+      const double z1 = h * (dx * (double)ay.x - dy * (double)ax.x);
+      V* dz = out + o + 2 * cs;
+      V u;
       if (add) {
-        const V b = out[o + cc * cs];
-        v.x = (T)((double)b.x + w[2 * cc]);
-        v.y = (T)((double)b.y + w[2 * cc + 1]);
+        const V b = *dz;
+        u.x = (T)((double)b.x + z0);
+        u.y = (T)((double)b.y + z1);
       } else {
-        v.x = (T)w[2 * cc];
-        v.y = (T)w[2 * cc + 1];
+        u.x = (T)z0;
+        u.y = (T)z1;
       }
-      out[o + cc * cs] = v;
+      *dz = u;
     }
   }
 }
@@ -1364,7 +1383,8 @@ void launch_curl_cheb(const void* in, void* out, int Nx, int Ny, int Nz, int Nm,
                       const double* kys, double H, double scale, int add, hipStream_t s)
 {
   typedef typename C2<T>::t V;
-  hipLaunchKernelGGL((k_curl_cheb<T>), dim3((Nm + 255) / 256), dim3(256), 0, s, (const V*)in, (V*)out, Nx, Ny, Nz,
+  const int nt = 4 * Nm;
+  hipLaunchKernelGGL((k_curl_cheb<T>), dim3((nt + 255) / 256), dim3(256), 0, s, (const V*)in, (V*)out, Nx, Ny, Nz,
                      Nm, kxs, kys, 2.0 / H, scale, add);
 }
 template void launch_curl_cheb<float>(const void*, void*, int, int, int, int, const double*, const double*, double,

@@ -212,7 +212,8 @@ __global__ void __launch_bounds__(256) k_zinv_mma(ZArgs A, const T* __restrict__
           else s += cc[1] * e2 + cc[2] * (z * e1) + cc[3] * (z * e2);
           val += (T)(s * A.inv_nxy);
         }
-        dst[(size_t)jj * ncols + col] = val;
+        if (A.accumulate) dst[(size_t)jj * ncols + col] += val;
+        else dst[(size_t)jj * ncols + col] = val;
       }
     }
   }
@@ -277,6 +278,14 @@ bool launch_zinv_mma(const ZArgs& A, const void* in, void* out, hipStream_t s)
   }
 }
 
+template <class T>
+bool zgemm_supported(const ZArgs& A)
+{
+  return zmma_tile<T>(2 * up4(A.NH)) > 0 && zmma_tile<T>(up4(A.NE) + up4(A.NO)) > 0;
+}
+
+template bool zgemm_supported<float>(const ZArgs&);
+template bool zgemm_supported<double>(const ZArgs&);
 template bool launch_zfwd_mma<float>(const ZArgs&, const void*, void*, hipStream_t);
 template bool launch_zfwd_mma<double>(const ZArgs&, const void*, void*, hipStream_t);
 template bool launch_zinv_mma<float>(const ZArgs&, const void*, void*, hipStream_t);

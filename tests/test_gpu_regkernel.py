@@ -41,12 +41,12 @@ def oracle(case):
     return o.OracleDPStokes(geom, Lx, Ly, 0.0, Lz, nx, ny, nz, ETA, 6, 1.785 * 6, **kw)
 
 
-def gpu(case, precision, reg=True):
+def gpu(case, precision, reg=True, solver="auto"):
     from doublyperiodicstokes_amd import DPStokes, StokesParameters
     name, geom, Lx, Ly, Lz, nx, ny, nz, N, wd = case
     par = StokesParameters(viscosity=ETA, Lx=Lx, Ly=Ly, zmin=0.0, zmax=Lz, w=6, w_d=wd, beta=1.785 * 6,
                            beta_d=1.785 * wd, nx=nx, ny=ny, nz=nz, mode=geom, precision=precision,
-                           useRegKernel=reg)
+                           useRegKernel=reg, solver=solver)
     s = DPStokes()
     s.initialize(par, N)
     return s
@@ -72,6 +72,26 @@ def test_regkernel_mdot_matches_oracle(case, precision):
     ref.set_positions(pos)
     v0_ref, _ = ref.mdot(F, None)
     assert rel_l2(v0, v0_ref) <= tol
+    s.clear()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("case", CASES[1:3], ids=[c[0] for c in CASES[1:3]])
+def test_regkernel_lane_solver_matches_oracle(case, precision):
+    """The same with the one-lane spectral chain (curls between the z
+    transforms and the BVP kernels instead of around the fused solve)."""
+    say("case %s %s regular torque kernel mdot, lane solver" % (case[0], precision))
+    pos, F, T = inputs(case, 2)
+    ref = oracle(case)
+    ref.set_positions(pos)
+    v_ref, w_ref = ref.mdot(F, T)
+    s = gpu(case, precision, solver="lane")
+    s.setPositions(pos)
+    v, w = s.Mdot(F, T)
+    tol = 1e-11 if precision == "double" else 1e-5
+    ev, ew = rel_l2(v, v_ref), rel_l2(w, w_ref)
+    say("  rel-L2 U = %.3e  W = %.3e" % (ev, ew))
+    assert ev <= tol and ew <= tol
     s.clear()
 
 
