@@ -344,6 +344,8 @@ int build_tiles(dps_handle* h, int wmax)
     // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
     SpreadArgs S{};
     S.TX = h->TX; S.TY = h->TY; S.ZC = h->ZC; S.rs_max = std::max(h->rs_m, h->rs_d);
+    if (h->has_dip && !h->reg)   // paired force-torque entries
+      S.rs_max = std::max(S.rs_max, h->km.wx + h->km.wy + h->km.wzmax + 2 * (h->kd.wx + h->kd.wy + h->kd.wzmax) + 7);
     if (spread_npf(S.rs_max) == 0) return fail(DPS_ERR_ARG, "kernel z support too wide for the spreading records");
     while (spread_lds_bytes(S, h->tsz) > 65536 && S.ZC > 1) S.ZC /= 2;
     if (spread_lds_bytes(S, h->tsz) > 65536) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
@@ -654,6 +656,26 @@ SpreadArgs spread_args(dps_handle* h)
   A.grid = h->d_grid;
   A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
   A.zr_dip = 0;
+  A.pair = 0;
+  A.ho = A.rs_max;
+  return A;
+}
+
+// force and torque of a particle as one spreading entry: possible when the two
+// kernels have the same x/y supports (derivative torque kernels)
+bool spread_pairable(const dps_handle* h)
+{
+  return h->has_dip && !h->reg && h->km.wx == h->kd.wx && h->km.wy == h->kd.wy;
+}
+
+SpreadArgs spread_args_pair(dps_handle* h)
+{
+  SpreadArgs A = spread_args(h);
+  A.pair = 1;
+  A.pm = h->km.wx + h->km.wy + h->km.wzmax;
+  A.pd = 2 * h->kd.wx + 2 * h->kd.wy + 2 * h->kd.wzmax;
+  A.ho = A.pm + A.pd;
+  A.rs_max = A.ho + 7;   // staged words: ho + 14
   return A;
 }
 
@@ -765,6 +787,8 @@ int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_
   if (h->reg) {
     spread_pass(spread_args(h), F, nullptr, mixed(h), dtype, s);
     if (Tq) spread_pass(spread_args_torque(h), Tq, nullptr, mixed(h), dtype, s);
+  } else if (Tq && spread_pairable(h)) {
+    spread_pass(spread_args_pair(h), F, Tq, mixed(h), dtype, s);
   } else {
     spread_pass(spread_args(h), F, Tq, mixed(h), dtype, s);
   }

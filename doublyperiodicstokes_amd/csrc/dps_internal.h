@@ -70,6 +70,12 @@ struct SpreadArgs {
   void* grid;   // [3][Nz][Ny][Nx] grid precision
   int zr_dip;   // kind-0 entries cull by the dipole z range (a pass that spreads torques
                 // with the torque kernel's records given as km/sup_m/rec_m)
+  // paired entries: force and torque of a particle in one entry (the two
+  // kernels share their x/y supports); staged words: the spreading part of
+  // the force record (pm words), of the torque record (pd words), then the
+  // header at ho: both supports, the force, the torque
+  int pair, pm, pd;
+  int ho;       // header offset in a staged entry
 };
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
 // words staged per thread and spreading entry for records of rs_max words (0: too long)

@@ -344,13 +344,34 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   }
   __syncthreads();
   const int ncand = tpre[ntl];
-  const int HO = A.rs_max;   // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2
+  // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2; paired
+  // entries: force support, torque support, force, torque
+  const int HO = A.ho;
   // staging thread (ss, su) loads words su, su+16, ... of entry ss of a batch
   const int ss = threadIdx.x >> 4, su = threadIdx.x & 15;
   auto prefetch = [&](int qb, int nb, T (&v)[NPF]) {
     if (ss >= nb) return;
     const int ent = queue[qb + ss];
     const int p = ent >> 1, kind = ent & 1;
+    if (A.pair) {
+      const T* rm = (const T*)A.rec_m + (size_t)p * A.rs_m;
+      const T* rd = (const T*)A.rec_d + (size_t)p * A.rs_d;
+      const int* sm = reinterpret_cast<const int*>(A.sup_m + p);
+      const int* sd = reinterpret_cast<const int*>(A.sup_d + p);
+#pragma unroll
+      for (int k = 0; k < NPF; ++k) {
+        const int off = su + 16 * k;
+        T xv = (T)0;
+        if (off < A.pm) xv = rm[off];
+        else if (off < A.pm + A.pd) xv = rd[off - A.pm];
+        else if (off >= HO && off < HO + 4) xv = (T)sm[off - HO];
+        else if (off >= HO + 4 && off < HO + 8) xv = (T)sd[off - HO - 4];
+        else if (off >= HO + 8 && off < HO + 11) xv = (T)F[3 * (size_t)p + off - HO - 8];
+        else if (off >= HO + 11 && off < HO + 14) xv = (T)Tq[3 * (size_t)p + off - HO - 11];
+        v[k] = xv;
+      }
+      return;
+    }
     const int rs = kind ? A.rs_d : A.rs_m;
     const T* rec = (const T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * rs;
     const int* sp = reinterpret_cast<const int*>((kind ? A.sup_d : A.sup_m) + p);
@@ -373,9 +394,13 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       while (j + 1 < ntl && tpre[j + 1] <= ci) ++j;
       const int p = A.order[tstart[j] + (ci - tpre[j])];
       const int4 zr = A.zrange[p];
-      for (int kind = 0; kind < nkind; ++kind) {
+      for (int kind = 0; kind < (A.pair ? 1 : nkind); ++kind) {
         const bool zd = kind || A.zr_dip;
-        const int klo = zd ? zr.z : zr.x, khi = zd ? zr.w : zr.y;
+        int klo = zd ? zr.z : zr.x, khi = zd ? zr.w : zr.y;
+        if (A.pair && zr.z <= zr.w) {   // the union of both kernels' rows
+          klo = min(klo, zr.z);
+          khi = max(khi, zr.w);
+        }
         // rows klo..khi, and their periodic images when z is periodic
         bool hz = false;
         for (int sh = A.periodic_z ? -1 : 0; sh <= (A.periodic_z ? 1 : 0); ++sh)
@@ -407,11 +432,11 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       for (int si = wave; si < nb; si += nwaves) {
         const T* st = bst + si * SLOT;
         // the entry's header is the same for every lane: scalar registers
-        const int kind = __builtin_amdgcn_readfirstlane(queue[qb + si] & 1);
+        int kind = __builtin_amdgcn_readfirstlane(queue[qb + si] & 1);
         const int bx = __builtin_amdgcn_readfirstlane((int)st[HO]);
         const int by = __builtin_amdgcn_readfirstlane((int)st[HO + 1]);
-        const int klo = __builtin_amdgcn_readfirstlane((int)st[HO + 2]);
-        const int span = __builtin_amdgcn_readfirstlane((int)st[HO + 3]);
+        int klo = __builtin_amdgcn_readfirstlane((int)st[HO + 2]);
+        int span = __builtin_amdgcn_readfirstlane((int)st[HO + 3]);
         const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
         const int W = kind ? A.kd.wzmax : A.km.wzmax;
         const T* wxr = st;
@@ -419,7 +444,29 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
         const T* dxr = wyr + wy;
         const T* dyr = dxr + wx;
         const T* zr = kind ? (dyr + wy) : (wyr + wy);   // z weights of rows klo + q
-        const T f0 = st[HO + 4], f1 = st[HO + 5], f2 = st[HO + 6];
+        T f0 = st[HO + 4], f1 = st[HO + 5], f2 = st[HO + 6];
+        // paired entry: rows of the union of the force kernel's (klm, spm) and
+        // the torque kernel's (kld, spd), torque t
+        int om = 0, od = 0, spm = 0, spd = 0;
+        T t0 = 0, t1 = 0, t2 = 0;
+        const T* pd_w = st + A.pm;   // torque record: [wx][wy][dx][dy][z][dz]
+        if (A.pair) {
+          kind = 2;
+          const int klm = klo;
+          spm = span;
+          const int kld = __builtin_amdgcn_readfirstlane((int)st[HO + 6]);
+          spd = __builtin_amdgcn_readfirstlane((int)st[HO + 7]);
+          if (spd > 0) {
+            const int lo = (spm > 0) ? min(klm, kld) : kld;
+            const int hi = (spm > 0) ? max(klm + spm, kld + spd) : kld + spd;
+            klo = lo;
+            span = hi - lo;
+          }
+          om = klo - klm;
+          od = klo - kld;
+          f0 = st[HO + 8]; f1 = st[HO + 9]; f2 = st[HO + 10];
+          t0 = st[HO + 11]; t1 = st[HO + 12]; t2 = st[HO + 13];
+        }
         // rows klo + q (+ a period when z is periodic) inside this chunk:
         // q in [qa, qb2)
         for (int szp = A.periodic_z ? -1 : 0; szp <= (A.periodic_z ? 1 : 0); ++szp) {
@@ -449,7 +496,34 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               const int i = ix0 + a, j = iy0 + b;
               T* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
               const int q1 = qb2;
-              if (kind == 0) {
+              if (kind == 2) {
+                // force (monopole) and torque curl at each point, one atomic
+                // per component
+                const T* wxd = pd_w;
+                const T* wyd = wxd + wx;
+                const T* dxd = wyd + wy;
+                const T* dyd = dxd + wx;
+                const T* zAd = dyd + wy;
+                const T* zBd = zAd + A.kd.wzmax;
+                const T wm = wxr[i] * wyr[j];
+                const T ax = dxd[i] * wyd[j], ay = wxd[i] * dyd[j], az = wxd[i] * wyd[j];
+                const T h = (T)0.5;
+                for (int q = qa + phase; q < q1; q += P) {
+                  const int qm = q + om, qd = q + od;
+                  const T zm = (qm >= 0 && qm < spm) ? zr[qm] : (T)0;
+                  T zA = (T)0, zB = (T)0;
+                  if (qd >= 0 && qd < spd) {
+                    zA = zAd[qd];
+                    zB = zBd[qd];
+                  }
+                  const T w = wm * zm;
+                  const T gx = ax * zA, gy = ay * zA, gz = az * zB;
+                  T* d = dcol + q * PL;
+                  atomicAdd(d, w * f0 + h * (gy * t2 - gz * t1));
+                  atomicAdd(d + vol, w * f1 + h * (gz * t0 - gx * t2));
+                  atomicAdd(d + 2 * vol, w * f2 + h * (gx * t1 - gy * t0));
+                }
+              } else if (kind == 0) {
                 const T wxy = wxr[i] * wyr[j];
                 const T g0 = f0, g1 = f1, g2 = f2;
                 int q = qa + phase;
