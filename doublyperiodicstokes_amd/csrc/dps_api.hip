@@ -705,6 +705,7 @@ InterpArgs interp_args(dps_handle* h)
   A.fA_m = h->d_fA_m; A.fA_d = h->d_fA_d; A.fB_d = h->d_fB_d;
   A.grid = h->d_grid;
   A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
+  A.pair = 0;
   return A;
 }
 
@@ -993,7 +994,9 @@ int do_interp(dps_handle* h, void* vel, void* ang, int dtype, hipStream_t s)
       interp_pass(A, a, nullptr, mixed(h), dtype, s);
     }
   } else {
-    interp_pass(interp_args(h), vel, a, mixed(h), dtype, s);
+    InterpArgs A = interp_args(h);
+    A.pair = a && spread_pairable(h);
+    interp_pass(A, vel, a, mixed(h), dtype, s);
   }
   HIPCHK(hipGetLastError());
   return DPS_OK;

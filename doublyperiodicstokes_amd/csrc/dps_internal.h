@@ -92,6 +92,7 @@ struct InterpArgs {
   const double *fA_m, *fA_d, *fB_d;
   const void* grid;
   int periodic_z;   // rows of a support wrap modulo Nz
+  int pair;         // velocities and angular velocities in one pass (shared x/y supports)
 };
 
 // Half the curl of a Fourier-Chebyshev field [c][k][Nm] (x, y derivatives i kx,

@@ -627,6 +627,72 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
   const T* __restrict__ U = reinterpret_cast<const T*>(A.grid);
   const size_t cs = (size_t)A.Nz * A.Ny * A.Nx;
   const size_t zs = (size_t)A.Ny * A.Nx;
+  if (A.pair && ang) {
+    // both kernels over the union of their rows, each grid point read once
+    const int4 sm = A.sup_m[p], sd = A.sup_d[p];
+    const int wx = A.km.wx, wy = A.km.wy;
+    const T* rm = (const T*)A.rec_m + (size_t)p * A.rs_m;
+    const T* rd = (const T*)A.rec_d + (size_t)p * A.rs_d;
+    const T* mwx = rm;
+    const T* mwy = rm + wx;
+    const T* mzi = mwy + wy + A.km.wzmax;   // interpolation weights, rows sm.z + q
+    const T* dwx = rd;
+    const T* dwy = rd + wx;
+    const T* ddx = dwy + wy;
+    const T* ddy = ddx + wx;
+    const T* dzi = ddy + wy + 2 * A.kd.wzmax;   // value, then d/dz, rows sd.z + q
+    const T* ddzi = dzi + A.kd.wzmax;
+    int lo = sm.z, hi = sm.z + sm.w;
+    if (sd.w > 0) {
+      lo = (sm.w > 0) ? min(lo, sd.z) : sd.z;
+      hi = (sm.w > 0) ? max(hi, sd.z + sd.w) : sd.z + sd.w;
+    }
+    const int nz = hi - lo, om = lo - sm.z, od = lo - sd.z;
+    const int ncol = wx * wy;
+    const int npts = ncol * nz;
+    const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
+    T d[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) d[i] = 0;
+    for (int t = lane; t < npts; t += 64) {
+      int col, a;
+      const int q = fdiv(t, ncol, incol, &col);
+      const int b = fdiv(col, wx, iwx, &a);
+      int x = sm.x + a, y = sm.y + b;
+      x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+      y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+      int zr_ = lo + q;
+      if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+      const size_t idx = (size_t)zr_ * zs + (size_t)y * A.Nx + x;
+      const T v0 = U[idx], v1 = U[idx + cs], v2 = U[idx + 2 * cs];
+      const int qm = q + om, qd = q + od;
+      const T zm = (qm >= 0 && qm < sm.w) ? mzi[qm] : (T)0;
+      T zv = (T)0, zd = (T)0;
+      if (qd >= 0 && qd < sd.w) {
+        zv = dzi[qd];
+        zd = ddzi[qd];
+      }
+      const T w = mwx[a] * mwy[b] * zm;
+      const T gx = ddx[a] * dwy[b] * zv;
+      const T gy = dwx[a] * ddy[b] * zv;
+      const T gz = dwx[a] * dwy[b] * zd;
+      d[0] += w * v0; d[1] += w * v1; d[2] += w * v2;
+      d[3] += gx * v0; d[4] += gx * v1; d[5] += gx * v2;
+      d[6] += gy * v0; d[7] += gy * v1; d[8] += gy * v2;
+      d[9] += gz * v0; d[10] += gz * v1; d[11] += gz * v2;
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) d[i] = wave_sum(d[i]);
+    if (lane == 0) {
+      vel[3 * p] = (Tout)d[0];
+      vel[3 * p + 1] = (Tout)d[1];
+      vel[3 * p + 2] = (Tout)d[2];
+      ang[3 * p] = (Tout)((T)(-0.5) * (d[8] - d[10]));
+      ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[9] - d[5]));
+      ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[4] - d[6]));
+    }
+    return;
+  }
   for (int kind = 0; kind < (ang ? 2 : 1); ++kind) {
     const int4 sup = kind ? A.sup_d[p] : A.sup_m[p];
     const KernelDesc& K = kind ? A.kd : A.km;
