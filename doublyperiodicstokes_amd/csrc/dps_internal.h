@@ -216,13 +216,15 @@ constexpr int spectral_modes_per_block(int lpm, int e, bool mixed)
   while (G > 1 && spectral_lds_bytes(lpm, e, G, mixed) > 65536) G /= 2;
   return G;
 }
-// waves per SIMD that the LDS of one CU (160 KiB) allows
+// waves per SIMD that the LDS of one CU (160 KiB) allows, at most 3: the
+// lane-group solve needs 136 (E = 2) to 226 (E = 8) VGPRs, and a tighter
+// register cap sends it to scratch
 constexpr int spectral_waves_per_simd(int lpm, int e, bool mixed)
 {
   const int G = spectral_modes_per_block(lpm, e, mixed);
   const int blocks = (int)(163840 / spectral_lds_bytes(lpm, e, G, mixed));
   const int w = blocks * (G * lpm / 64) / 4;
-  return w < 1 ? 1 : (w > 8 ? 8 : w);
+  return w < 1 ? 1 : (w > 3 ? 3 : w);
 }
 void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
                        double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,

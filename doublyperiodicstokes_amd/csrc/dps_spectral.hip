@@ -175,6 +175,24 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
     S[row * GS + gc] = v;
   }
   __syncthreads();
+  // fold node pairs in place, once per column: row j < Nz-1-j takes
+  // x_j + x_{Nz-1-j} and row Nz-1-j takes x_j - x_{Nz-1-j} (the middle node
+  // of an odd Nz stays; its odd part is a zero padding row below)
+  {
+    const int np = Nz / 2;
+    for (int i = threadIdx.x; i < 3 * np * G; i += NT) {
+      const int gg = i % G, cj = i / G;
+      const int c = cj / np, j = cj - c * np;
+      V& a = S[(c * NZP + j) * GS + gg];
+      V& b = S[(c * NZP + Nz - 1 - j) * GS + gg];
+      const V x = a, y = b;
+      a.x = x.x + y.x;
+      a.y = x.y + y.y;
+      b.x = x.x - y.x;
+      b.y = x.y - y.y;
+    }
+  }
+  __syncthreads();
 
   // ---- phase 1: forward transform of this lane's rows ----
   {
@@ -189,13 +207,16 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
 #pragma unroll 2
     for (int j = 0; j < NH; ++j) {
       T er[3], ei[3], orr[3], oi[3];
-      const bool mid = (j == Nz - 1 - j);
+      // NZP is even, so an odd Nz leaves row NZP-1 as zero padding
+      const int jo = (j == Nz - 1 - j) ? NZP - 1 : Nz - 1 - j;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const V a = S[(c * NZP + j) * GS + g];
-        const V b = S[(c * NZP + Nz - 1 - j) * GS + g];
-        if (mid) { er[c] = a.x; ei[c] = a.y; orr[c] = oi[c] = (T)0; }
-        else { er[c] = a.x + b.x; ei[c] = a.y + b.y; orr[c] = a.x - b.x; oi[c] = a.y - b.y; }
+        const V b = S[(c * NZP + jo) * GS + g];
+        er[c] = a.x;
+        ei[c] = a.y;
+        orr[c] = b.x;
+        oi[c] = b.y;
       }
       T t[E];
 #pragma unroll
