@@ -163,16 +163,32 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   const int mc = (m0 + gc < Nm) ? m0 + gc : -1;
 
   // ---- phase 0: row tables and columns in ----
-  for (int i = threadIdx.x; i < TB_N * NZP; i += NT) {
-    const int t = i / NZP, row = i - t * NZP;
-    tab[i] = (row < Nz) ? row_table(A.B, t)[row] : 0.0;
-  }
-  for (int row = threadIdx.x / G; row < 3 * NZP; row += NT / G) {
-    const int c = row / NZP, r = row - c * NZP;
-    V v;
-    v.x = v.y = (T)0;
-    if (mc >= 0 && r < Nz) v = in[(size_t)(c * Nz + r) * Nm + mc];
-    S[row * GS + gc] = v;
+  // every load of a thread is issued before the first LDS write, so the
+  // workgroup waits for HBM once rather than once per row
+  {
+    constexpr int NTB = (TB_N * NZP + NT - 1) / NT;
+    double tv[NTB];
+#pragma unroll
+    for (int k = 0; k < NTB; ++k) {
+      const int i = threadIdx.x + k * NT;
+      const int t = i / NZP, row = i - t * NZP;
+      tv[k] = (i < TB_N * NZP && row < Nz) ? row_table(A.B, t)[row] : 0.0;
+    }
+    // thread rows threadIdx.x / G + k LPM, k < 3 E, cover the 3 NZP rows
+    constexpr int NL = 3 * E;
+    V cv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int row = threadIdx.x / G + k * LPM;
+      const int c = row / NZP, r = row - c * NZP;
+      cv[k].x = cv[k].y = (T)0;
+      if (mc >= 0 && r < Nz) cv[k] = in[(size_t)(c * Nz + r) * Nm + mc];
+    }
+#pragma unroll
+    for (int k = 0; k < NTB; ++k)
+      if (threadIdx.x + k * NT < TB_N * NZP) tab[threadIdx.x + k * NT] = tv[k];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) S[(threadIdx.x / G + k * LPM) * GS + gc] = cv[k];
   }
   __syncthreads();
   // fold node pairs in place, once per column: row j < Nz-1-j takes
@@ -204,10 +220,11 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
       for (int q = 0; q < E; ++q) ar[c][q] = ai[c][q] = (T)0;
     // [NH][LPM][E]: this lane's E rows of node pair j are contiguous
     const T2* Tf = (const T2*)A.TfP + l * (E / 2);
-#pragma unroll 2
+    // NZP is even, so an odd Nz leaves row NZP-1 as zero padding for the
+    // middle node's odd part
+#pragma unroll 4
     for (int j = 0; j < NH; ++j) {
       T er[3], ei[3], orr[3], oi[3];
-      // NZP is even, so an odd Nz leaves row NZP-1 as zero padding
       const int jo = (j == Nz - 1 - j) ? NZP - 1 : Nz - 1 - j;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -317,7 +334,7 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < R; ++r) er[c][r] = ei[c][r] = orr[c][r] = oi[c][r] = (T)0;
-#pragma unroll 2
+#pragma unroll 4
     for (int p = 0; p < Nz; p += 2) {
       V u[3], w[3];
 #pragma unroll
