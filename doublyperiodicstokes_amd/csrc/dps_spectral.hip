@@ -329,6 +329,14 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
     // [ceil(Nz/2)][LPM][E]: for the coefficient pair (2p, 2p+1), the R rows
     // of coefficient 2p then the R rows of 2p+1, for this lane's node pairs
     const T2* Ti = (const T2*)A.TiP + l * (E / 2);
+    // coordinates of this lane's node pairs, loaded ahead of the sums
+    double zj[R], zm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = (l + LPM * r < NH) ? l + LPM * r : NH - 1;
+      zj[r] = A.zpts[j];
+      zm[r] = A.zpts[Nz - 1 - j];
+    }
     T er[3][R], ei[3][R], orr[3][R], oi[3][R];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -370,16 +378,22 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
       if (j >= NH) continue;
       const int jm = Nz - 1 - j;
       // z_{Nz-1-j} = Lz - z_j, so e^{K(z - Lz)} at one node of the pair is
-      // e^{-K z} at the other
+      // e^{-K z} at the other.  The profile lands in grid precision, so an
+      // FP32 grid takes FP32 exponentials (the coefficients stay FP64).
       double ej = 0, em = 0;
       if (nb) {
-        ej = exp(-K * A.zpts[j]);
-        em = (jm == j) ? ej : exp(-K * A.zpts[jm]);
+        if constexpr (sizeof(T) == 4) {
+          ej = (double)expf((float)(-K * zj[r]));
+          em = (jm == j) ? ej : (double)expf((float)(-K * zm[r]));
+        } else {
+          ej = exp(-K * zj[r]);
+          em = (jm == j) ? ej : exp(-K * zm[r]);
+        }
       }
       for (int side = 0; side < 2; ++side) {
         const int jj = side ? jm : j;
         if (side && jj == j) break;
-        const double z = A.zpts[jj];
+        const double z = side ? zm[r] : zj[r];
         const double e1 = side ? em : ej;
         const double e2 = side ? ej : em;
 #pragma unroll
@@ -409,11 +423,25 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   __syncthreads();
 
   // ---- phase 4: columns out ----
-  if (mc >= 0)
-    for (int row = threadIdx.x / G; row < 3 * Nz; row += NT / G) {
-      const int c = row / Nz, r = row - c * Nz;
-      out[(size_t)row * Nm + mc] = S[(c * NZP + r) * GS + gc];
+  // (the LDS rows of a thread are read together, then stored; the thread
+  // index is re-read opaquely so the row offsets are not shared with phase 0
+  // and kept live through the solve)
+  {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int gc4 = tid % G;
+    const int mc4 = (m0 + gc4 < Nm) ? m0 + gc4 : -1;
+    constexpr int NL = 3 * E;
+    V cv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) cv[k] = S[(tid / G + k * LPM) * GS + gc4];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int row = tid / G + k * LPM;
+      const int c = row / NZP, r = row - c * NZP;
+      if (mc4 >= 0 && r < Nz) out[(size_t)(c * Nz + r) * Nm + mc4] = cv[k];
     }
+  }
 }
 
 // Static factorisation of every mode (depends on |k| only), one lane per
