@@ -28,13 +28,13 @@ template <class T, int N> struct FftTile {
 };
 
 template <class T, int R1, int R2, int DIR>
-__global__ void __launch_bounds__(256) k_fft_y(typename C2<T>::t* __restrict__ data, int ncols, int Nkx,
+__global__ void __launch_bounds__(512) k_fft_y(typename C2<T>::t* __restrict__ data, int ncols, int Nkx,
                                                const typename C2<T>::t* __restrict__ tw)
 {
   typedef typename C2<T>::t V;
   constexpr int N = R1 * R2;
   constexpr int TW = FftTile<T, N>::value;
-  constexpr int G1 = 256 / TW;   // column groups
+  constexpr int G1 = 512 / TW;   // column groups (512 threads: more loads in flight per CU)
   __shared__ V buf[N * TW];       // [n1][k2][col]
   const int col = threadIdx.x % TW, g = threadIdx.x / TW;
   // neighbouring tiles share the cache lines at their row ends (rows of Nkx
@@ -100,9 +100,9 @@ void fft_y_launch(void* data, int nslices, int Nkx, const void* tw, int dir, hip
   const int ncols = nslices * Nkx;
   const dim3 grid((ncols + TW - 1) / TW);
   if (dir < 0)
-    hipLaunchKernelGGL((k_fft_y<T, R1, R2, -1>), grid, dim3(256), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
+    hipLaunchKernelGGL((k_fft_y<T, R1, R2, -1>), grid, dim3(512), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
   else
-    hipLaunchKernelGGL((k_fft_y<T, R1, R2, 1>), grid, dim3(256), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
+    hipLaunchKernelGGL((k_fft_y<T, R1, R2, 1>), grid, dim3(512), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
 }
 
 }  // namespace
