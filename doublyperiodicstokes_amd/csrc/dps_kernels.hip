@@ -71,23 +71,86 @@ __global__ void k_advance(Tp* __restrict__ pos, const Tp* __restrict__ vel, doub
   pos[3 * p + 2] = (Tp)z;
 }
 
-template <class Tin, class T>
-__global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
+// reductions over the 16-lane group of a particle
+__device__ __forceinline__ int grp_min(int v)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.N) return;
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 16));
+  return v;
+}
+__device__ __forceinline__ int grp_max(int v)
+{
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 16));
+  return v;
+}
+__device__ __forceinline__ int grp_sum(int v)
+{
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o, 16);
+  return v;
+}
+
+// stencil_support with the two searches of the descending extended z nodes
+// done by the lane group at once: the first entry <= v of a descending array
+// is the number of entries > v, counted over the lanes (no chain of
+// dependent loads)
+__device__ __forceinline__ Stencil grp_stencil_support(double x, double y, double z, double hx, double hy,
+                                                       const KernelDesc& k, const double* zext, int sl)
+{
+  Stencil s;
+  s.bx = support_start(x, hx, k.wx);
+  s.by = support_start(y, hy, k.wy);
+  const double vh = z - k.thresh, vl = z + k.thresh;
+  int ch = 0, cl = 0;
+  for (int e = sl; e < k.nzeff; e += 16) {
+    const double ze = zext[e];
+    ch += ze > vh;
+    cl += ze > vl;
+  }
+  const int high = grp_sum(ch), low = grp_sum(cl);
+  int ir = high;
+  if (ir == k.nzeff) ir -= 1;
+  else if (vh > zext[ir]) ir -= 1;
+  s.e0 = low;
+  s.nz = ir - low + 1;
+  if (s.nz < 0) s.nz = 0;
+  return s;
+}
+
+// Supports and weight records, four particles per wavefront (16 lanes each).
+// Lanes form the x and y weights (lane i: point i) and the contributions of
+// the extended z support (lane q: entry q, kept in LDS with the interior row
+// it folds onto); then lane r sums the contributions of interior row r in
+// entry order, in grid precision, as a serial pass adding them to a zeroed
+// record would.  nq bounds the entries of a z support (the kernels' largest
+// window).
+template <class Tin, class T>
+__global__ void __launch_bounds__(64) k_stencil(StencilArgs A, const Tin* __restrict__ pos_in, int nq)
+{
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int sl = threadIdx.x & 15, pg = threadIdx.x >> 4;
+  double* zv = reinterpret_cast<double*>(smem_raw) + pg * 4 * nq;    // [4][nq] contributions of each entry
+  int* zrow = reinterpret_cast<int*>(reinterpret_cast<double*>(smem_raw) + 16 * nq) + pg * nq;   // [nq] row, -1: none
+  const int p0 = blockIdx.x * 4 + pg;
+  const bool valid = p0 < A.N;   // the last block's spare groups read particle N-1 and write nothing
+  const int p = valid ? p0 : A.N - 1;
+  const bool lead = valid && sl == 0;
   // periodic wrap in x, y (ParticleList.cpp:561-580)
   double x = wrap_periodic((double)pos_in[3 * p] - A.xmin, A.Lx);
   double y = wrap_periodic((double)pos_in[3 * p + 1] - A.ymin, A.Ly);
   double z = (double)pos_in[3 * p + 2] - A.zmin;
   if (A.periodic_z) z = wrap_periodic(z, A.Lz);   // ParticleList.cpp:583-595
   if (!(z >= 0 && z <= A.Lz)) {
-    atomicOr(A.err, 1);
+    if (lead) atomicOr(A.err, 1);
     z = z < 0 ? 0.0 : A.Lz;   // keep the stencil in range; the call reports an error
   }
-  A.pos[3 * p] = x;
-  A.pos[3 * p + 1] = y;
-  A.pos[3 * p + 2] = z;
+  if (lead) {
+    A.pos[3 * p] = x;
+    A.pos[3 * p + 1] = y;
+    A.pos[3 * p + 2] = z;
+  }
+  int4 zr = A.zrange[p];
   const double hxhy = A.hx * A.hy;
   for (int kind = A.first_kind; kind < (A.has_dip ? 2 : 1); ++kind) {
     // deriv: the torque kernel's derivative records; a regular torque kernel
@@ -96,6 +159,7 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
     const KernelDesc& K = kind ? A.kd : A.km;
     const double* zext = kind ? A.zext_d : A.zext_m;
     const double* zwe = kind ? A.zw_d : A.zw_m;
+    const double a2 = K.alpha * K.alpha;
     Stencil s;
     if (A.periodic_z) {
       s.bx = support_start(x, A.hx, K.wx);
@@ -103,113 +167,128 @@ __global__ void k_stencil(StencilArgs A, const Tin* __restrict__ pos_in)
       s.e0 = 0;
       s.nz = 0;
     } else {
-      s = stencil_support(x, y, z, A.hx, A.hy, K, zext);
+      s = grp_stencil_support(x, y, z, A.hx, A.hy, K, zext, sl);
       if (s.nz > K.wzmax) s.nz = K.wzmax;   // cannot happen (wzmax bounds the window)
     }
-    int4 sup;
-    sup.x = s.bx; sup.y = s.by; sup.z = s.e0; sup.w = s.nz;
     T* rec = (T*)(kind ? A.rec_d : A.rec_m) + (size_t)p * (kind ? A.rs_d : A.rs_m);
-    double wx[DPS_MAX_W], dx[DPS_MAX_W], wy[DPS_MAX_W], dy[DPS_MAX_W];
-    stencil_weights_xy(x, A.hx, s.bx, K.wx, K, K.inv_norm3, wx, deriv ? dx : nullptr);
-    stencil_weights_xy(y, A.hy, s.by, K.wy, K, 1.0, wy, deriv ? dy : nullptr);
-    int o = 0;
-    for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)wx[i];
-    o += K.wx;
-    for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)wy[j];
-    o += K.wy;
-    if (deriv) {
-      for (int i = 0; i < K.wx; ++i) rec[o + i] = (T)dx[i];
-      o += K.wx;
-      for (int j = 0; j < K.wy; ++j) rec[o + j] = (T)dy[j];
-      o += K.wy;
+    // x and y weights (1/norm^3 folded into x), derivative weights after them
+    if (valid) {
+      for (int i = sl; i < K.wx; i += 16) {
+        const double v = clamp_off((double)(s.bx + i) * A.hx - x, K.thresh);
+        const double ph = es_phi(v, K.betaw, a2) * K.inv_norm3;
+        rec[i] = (T)ph;
+        if (deriv)
+          rec[K.wx + K.wy + i] = (T)((K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2));
+      }
+      for (int j = sl; j < K.wy; j += 16) {
+        const double v = clamp_off((double)(s.by + j) * A.hy - y, K.thresh);
+        const double ph = es_phi(v, K.betaw, a2) * 1.0;
+        rec[K.wx + j] = (T)ph;
+        if (deriv)
+          rec[2 * K.wx + K.wy + j] = (T)((K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2));
+      }
     }
+    const int o = deriv ? 2 * (K.wx + K.wy) : K.wx + K.wy;
+    const int W = K.wzmax;
+    int4 sup;
+    sup.x = s.bx;
+    sup.y = s.by;
     if (A.periodic_z) {
       // uniform periodic z (ParticleList.cpp:361-368): w_z = W points from
       // support_start, rows wrapped modulo Nz where used; interpolation
       // weight hx hy hz
-      const double a2 = K.alpha * K.alpha;
-      const int W = K.wzmax;
       const int bz = support_start(z, A.hz, W);
-      for (int q = 0; q < W; ++q) {
-        const double v = clamp_off((double)(bz + q) * A.hz - z, K.thresh);
-        const double ph = es_phi(v, K.betaw, a2);
-        const double qw = hxhy * A.hz;
-        if (!deriv) {
-          rec[o + q] = (T)ph;
-          rec[o + W + q] = (T)(ph * qw);
-        } else {
-          const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
-          rec[o + q] = (T)ph;
-          rec[o + W + q] = (T)d;
-          rec[o + 2 * W + q] = (T)(ph * qw);
-          rec[o + 3 * W + q] = (T)(d * qw);
+      if (valid)
+        for (int q = sl; q < W; q += 16) {
+          const double v = clamp_off((double)(bz + q) * A.hz - z, K.thresh);
+          const double ph = es_phi(v, K.betaw, a2);
+          const double qw = hxhy * A.hz;
+          if (!deriv) {
+            rec[o + q] = (T)ph;
+            rec[o + W + q] = (T)(ph * qw);
+          } else {
+            const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
+            rec[o + q] = (T)ph;
+            rec[o + W + q] = (T)d;
+            rec[o + 2 * W + q] = (T)(ph * qw);
+            rec[o + 3 * W + q] = (T)(d * qw);
+          }
         }
-      }
       sup.z = bz;
       sup.w = W;
-      (kind ? A.sup_d : A.sup_m)[p] = sup;
-      int4 zr = A.zrange[p];
       if (kind) { zr.z = bz; zr.w = bz + W - 1; }
       else { zr.x = bz; zr.y = bz + W - 1; zr.z = 1 << 30; zr.w = -1; }
-      A.zrange[p] = zr;
+    } else {
+      // z support, wall images folded in and merged: dense weights of the
+      // interior rows klo .. klo + span - 1 (span <= W, since the direct rows
+      // and the folded images form one run of rows), each image added to the
+      // row it lands on with its fold factor (class A for the value kernel,
+      // class B for d/dz); open-end ghosts carry factor 0 and drop out
+      const int* kt = kind ? A.kt_d : A.kt_m;
+      const double* fa = kind ? A.fA_d : A.fA_m;
+      const double* fb = deriv ? A.fB_d : nullptr;
+      int lo = 1 << 30, hi = -1;
+      for (int q = sl; q < s.nz; q += 16) {
+        const int e = s.e0 + q;
+        const double sa = fa[e], sb = deriv ? fb[e] : 0.0;
+        int row = -1;
+        if (!(sa == 0.0 && sb == 0.0)) {
+          const double v = clamp_off(zext[e] - z, K.thresh);
+          const double ph = es_phi(v, K.betaw, a2);
+          const double qw = hxhy * zwe[e];
+          row = kt[e];
+          lo = min(lo, row);
+          hi = max(hi, row);
+          if (!deriv) {
+            zv[q] = ph * sa;                 // spread
+            zv[nq + q] = ph * qw * sa;       // interpolation
+          } else {
+            const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
+            zv[q] = ph * sa;
+            zv[nq + q] = d * sb;
+            zv[2 * nq + q] = ph * qw * sa;
+            zv[3 * nq + q] = d * qw * sb;
+          }
+        }
+        zrow[q] = row;
+      }
+      const int klo = grp_min(lo), khi = grp_max(hi);
+      __syncthreads();
+      if (valid)
+        for (int r = sl; r < W; r += 16) {
+          T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0;
+          for (int q = 0; q < s.nz; ++q) {
+            if (zrow[q] != klo + r) continue;
+            c0 += (T)zv[q];
+            c1 += (T)zv[nq + q];
+            if (deriv) {
+              c2 += (T)zv[2 * nq + q];
+              c3 += (T)zv[3 * nq + q];
+            }
+          }
+          rec[o + r] = c0;
+          rec[o + W + r] = c1;
+          if (deriv) {
+            rec[o + 2 * W + r] = c2;
+            rec[o + 3 * W + r] = c3;
+          }
+        }
+      __syncthreads();   // the entries' LDS is reused by the next kernel kind
+      sup.z = (khi >= klo) ? klo : 0;
+      sup.w = (khi >= klo) ? khi - klo + 1 : 0;
+      if (kind) { zr.z = klo; zr.w = khi; }
+      else { zr.x = klo; zr.y = khi; zr.z = 1 << 30; zr.w = -1; }
+    }
+    if (lead) {
+      (kind ? A.sup_d : A.sup_m)[p] = sup;
       if (kind == 0) {
+        // tile key from the monopole support centre
         const int cx = wrapi(s.bx + K.wx / 2, A.Nx), cy = wrapi(s.by + K.wy / 2, A.Ny);
         A.key[p] = (cy / A.TY) * A.ntx + (cx / A.TX);
       }
-      continue;
-    }
-    // z support, wall images folded in and merged: dense weights of the
-    // interior rows klo .. klo + span - 1 (span <= W, since the direct rows
-    // and the folded images form one run of rows), each image added to the
-    // row it lands on with its fold factor (class A for the value kernel,
-    // class B for d/dz); open-end ghosts carry factor 0 and drop out
-    const double a2 = K.alpha * K.alpha;
-    const int W = K.wzmax;
-    const int* kt = kind ? A.kt_d : A.kt_m;
-    const double* fa = kind ? A.fA_d : A.fA_m;
-    const double* fb = deriv ? A.fB_d : nullptr;
-    int klo = 1 << 30, khi = -1;
-    for (int q = 0; q < s.nz; ++q) {
-      const int e = s.e0 + q;
-      if (fa[e] == 0.0 && (deriv ? fb[e] : 0.0) == 0.0) continue;
-      klo = min(klo, kt[e]);
-      khi = max(khi, kt[e]);
-    }
-    for (int i = 0; i < (deriv ? 4 : 2) * W; ++i) rec[o + i] = (T)0;
-    for (int q = 0; q < s.nz; ++q) {
-      const int e = s.e0 + q;
-      const double sa = fa[e], sb = deriv ? fb[e] : 0.0;
-      if (sa == 0.0 && sb == 0.0) continue;
-      const double v = clamp_off(zext[e] - z, K.thresh);
-      const double ph = es_phi(v, K.betaw, a2);
-      const double qw = hxhy * zwe[e];
-      const int r = kt[e] - klo;
-      if (!deriv) {
-        rec[o + r] += (T)(ph * sa);                 // spread
-        rec[o + W + r] += (T)(ph * qw * sa);        // interpolation
-      } else {
-        const double d = (K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2);
-        rec[o + r] += (T)(ph * sa);
-        rec[o + W + r] += (T)(d * sb);
-        rec[o + 2 * W + r] += (T)(ph * qw * sa);
-        rec[o + 3 * W + r] += (T)(d * qw * sb);
-      }
-    }
-    sup.z = (khi >= klo) ? klo : 0;
-    sup.w = (khi >= klo) ? khi - klo + 1 : 0;
-    (kind ? A.sup_d : A.sup_m)[p] = sup;
-    {
-      int4 zr = A.zrange[p];
-      if (kind) { zr.z = klo; zr.w = khi; }
-      else { zr.x = klo; zr.y = khi; zr.z = 1 << 30; zr.w = -1; }
-      A.zrange[p] = zr;
-    }
-    if (kind == 0) {
-      // tile key from the monopole support centre
-      const int cx = wrapi(s.bx + K.wx / 2, A.Nx), cy = wrapi(s.by + K.wy / 2, A.Ny);
-      A.key[p] = (cy / A.TY) * A.ntx + (cx / A.TX);
     }
   }
+  if (lead) A.zrange[p] = zr;
 }
 
 __global__ void k_reset_err(int* e) { *e = 0; }
@@ -1362,8 +1441,10 @@ template <class Tin, class T>
 void launch_stencil(const StencilArgs& A, const void* pos, hipStream_t s)
 {
   if (A.N == 0) return;
-  hipLaunchKernelGGL((k_stencil<Tin, T>), dim3((A.N + 127) / 128), dim3(128), 0, s, A,
-                     (const Tin*)pos);
+  // four particles per 64-lane block, each with [4][nq] doubles and [nq] ints of LDS
+  const int nq = std::max(1, std::max(A.km.wzmax, A.has_dip ? A.kd.wzmax : 0));
+  hipLaunchKernelGGL((k_stencil<Tin, T>), dim3((A.N + 3) / 4), dim3(64),
+                     (size_t)4 * nq * (4 * sizeof(double) + sizeof(int)), s, A, (const Tin*)pos, nq);
 }
 
 template <class Tp>
