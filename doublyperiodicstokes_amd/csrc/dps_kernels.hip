@@ -1,7 +1,8 @@
 // dps_kernels.hip -- gfx950 kernels of the doubly periodic Stokes Mdot path.
 //
 //  k_stencil      wrap/shift positions, z-range check, per-particle support and
-//                 1-D ES weights (ParticleList.cpp:394-548), tile key
+//                 1-D ES weights (ParticleList.cpp:394-548), tile key; four
+//                 particles per wavefront, 16 lanes each
 //  k_tile_*       counting sort of particles by x-y tile
 //  k_spread       forces + torque curl onto the grid: one workgroup per
 //                 (x-y tile, z chunk), LDS accumulation, wall images folded on
