@@ -137,7 +137,7 @@ DPS_HD void stencil_weights_xy(double pos, double h, int b, int w, const KernelD
 {
   const double a2 = k.alpha * k.alpha;
   for (int i = 0; i < w; ++i) {
-    const double v = clamp_off((double)(b + i) * h - pos, k.thresh);
+    const double v = clamp_off(fma((double)(b + i), h, -pos), k.thresh);
     const double p = es_phi(v, k.betaw, a2) * scale;
     if (wout) wout[i] = p;
     if (dout) dout[i] = (k.pos_deriv && fabs(v) >= k.thresh) ? 0.0 : p * es_dfac(v, k.betaw, a2);

@@ -175,14 +175,14 @@ __global__ void __launch_bounds__(64) k_stencil(StencilArgs A, const Tin* __rest
     // x and y weights (1/norm^3 folded into x), derivative weights after them
     if (valid) {
       for (int i = sl; i < K.wx; i += 16) {
-        const double v = clamp_off((double)(s.bx + i) * A.hx - x, K.thresh);
+        const double v = clamp_off(fma((double)(s.bx + i), A.hx, -x), K.thresh);
         const double ph = es_phi(v, K.betaw, a2) * K.inv_norm3;
         rec[i] = (T)ph;
         if (deriv)
           rec[K.wx + K.wy + i] = (T)((K.pos_deriv && fabs(v) >= K.thresh) ? 0.0 : ph * es_dfac(v, K.betaw, a2));
       }
       for (int j = sl; j < K.wy; j += 16) {
-        const double v = clamp_off((double)(s.by + j) * A.hy - y, K.thresh);
+        const double v = clamp_off(fma((double)(s.by + j), A.hy, -y), K.thresh);
         const double ph = es_phi(v, K.betaw, a2) * 1.0;
         rec[K.wx + j] = (T)ph;
         if (deriv)
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(64) k_stencil(StencilArgs A, const Tin* __rest
       const int bz = support_start(z, A.hz, W);
       if (valid)
         for (int q = sl; q < W; q += 16) {
-          const double v = clamp_off((double)(bz + q) * A.hz - z, K.thresh);
+          const double v = clamp_off(fma((double)(bz + q), A.hz, -z), K.thresh);
           const double ph = es_phi(v, K.betaw, a2);
           const double qw = hxhy * A.hz;
           if (!deriv) {
@@ -942,7 +942,7 @@ __global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* _
     const int wx = K.wx;
     for (int i = 0; i < wx; ++i) {
       double w, df, dp, dd;
-      axis((double)(sup.x + i) * A.hx - x, K.inv_norm3, &w, &df, &dp, &dd);
+      axis(fma((double)(sup.x + i), A.hx, -x), K.inv_norm3, &w, &df, &dp, &dd);
       rec[o + i] = (T)w;
       rec[o + wx + i] = (T)df;
       rec[o + 2 * wx + i] = (T)dp;
@@ -952,7 +952,7 @@ __global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* _
     const int wy = K.wy;
     for (int j = 0; j < wy; ++j) {
       double w, df, dp, dd;
-      axis((double)(sup.y + j) * A.hy - y, 1.0, &w, &df, &dp, &dd);
+      axis(fma((double)(sup.y + j), A.hy, -y), 1.0, &w, &df, &dp, &dd);
       rec[o + j] = (T)w;
       rec[o + wy + j] = (T)df;
       rec[o + 2 * wy + j] = (T)dp;
@@ -966,7 +966,7 @@ __global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* _
   if (A.periodic_z) {
     for (int q = 0; q < sup.w; ++q) {
       double w, df, dp, dd;
-      axis((double)(sup.z + q) * A.hz - z, 1.0, &w, &df, &dp, &dd);
+      axis(fma((double)(sup.z + q), A.hz, -z), 1.0, &w, &df, &dp, &dd);
       const double qw = hxhy * A.hz;
       rec[o + q] = (T)(w * qw);
       rec[o + W + q] = (T)(dp * qw);

@@ -86,6 +86,9 @@ def lib():
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp, dp]
         L.orc_interp_pos.argtypes = [ctypes.c_int, ctypes.c_double] + L.orc_interp.argtypes[1:]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_fma_offsets.argtypes = [ctypes.c_int, ctypes.c_int, dp,
+                                      ctypes.POINTER(ctypes.c_longlong), ctypes.c_double,
+                                      ctypes.c_double, dp]
         _lib = L
     return _lib
 
@@ -228,15 +231,19 @@ def kbpenta_precompute(d0, dp2, dm2):
 def kbpenta_solve(beta, dp2, dm2, rhs):
     """Solve with stored beta; rhs (M, nz, nrhs) real or complex.
     LinearSolvers.h:76-141 (same recurrence for the real and complex solver)."""
-    M, nz = rhs.shape[0], rhs.shape[1]
-    x = np.array(rhs, copy=True)
+    nz = rhs.shape[1]
+    # rows first, so that every step of the recurrences reads contiguous memory
+    x = np.ascontiguousarray(np.moveaxis(rhs, 1, 0))
+    bT = np.ascontiguousarray(beta.T)[:, :, None]
+    pT = np.ascontiguousarray(dp2.T)[:, :, None]
+    mT = np.ascontiguousarray(dm2.T)[:, :, None]
     for i in range(2, nz):
-        x[:, i] = x[:, i] - (dm2[:, i - 2, None] * x[:, i - 2]) / beta[:, i - 1, None]
-    x[:, nz - 1] /= beta[:, nz, None]
-    x[:, nz - 2] /= beta[:, nz - 1, None]
+        x[i] = x[i] - (mT[i - 2] * x[i - 2]) / bT[i - 1]
+    x[nz - 1] /= bT[nz]
+    x[nz - 2] /= bT[nz - 1]
     for i in range(nz - 3, -1, -1):
-        x[:, i] = (x[:, i] - dp2[:, i, None] * x[:, i + 2]) / beta[:, i + 1, None]
-    return x
+        x[i] = (x[i] - pT[i] * x[i + 2]) / bT[i + 1]
+    return np.ascontiguousarray(np.moveaxis(x, 0, 1))
 
 
 # ----------------------------------------------------------------------------
@@ -291,15 +298,11 @@ def _banded_S(S, x, Nz):
     """SIMat @ x for x (M, Nz+2, ...) using its band (0, +-2) and extra columns."""
     out = np.zeros(x.shape[:1] + (Nz,) + x.shape[2:], dtype=x.dtype)
     idx = np.arange(Nz)
-    out += S[idx, idx][None, :, None] * x[:, :Nz] if x.ndim == 3 else S[idx, idx][None, :] * x[:, :Nz]
     lo = np.arange(2, Nz)
-    hi = np.arange(0, Nz)
-    if x.ndim == 3:
-        out[:, lo] += S[lo, lo - 2][None, :, None] * x[:, lo - 2]
-        out[:, hi] += S[hi, hi + 2][None, :, None] * x[:, hi + 2]
-    else:
-        out[:, lo] += S[lo, lo - 2][None, :] * x[:, lo - 2]
-        out[:, hi] += S[hi, hi + 2][None, :] * x[:, hi + 2]
+    ex = (None, slice(None)) + (None,) * (x.ndim - 2)
+    out += S[idx, idx][ex] * x[:, :Nz]
+    out[:, 2:Nz] += S[lo, lo - 2][ex] * x[:, 0:Nz - 2]
+    out[:, 0:Nz] += S[idx, idx + 2][ex] * x[:, 2:Nz + 2]
     # null-space columns (constant and linear terms): SIMat[0, N], SIMat[1, N+1]
     out[:, 0] += S[0, Nz] * x[:, Nz]
     out[:, 1] += S[1, Nz + 1] * x[:, Nz + 1]
@@ -311,8 +314,8 @@ def _banded_F(F, x, Nz):
     out = np.zeros((x.shape[0], Nz), dtype=x.dtype)
     lo = np.arange(1, Nz)
     hi = np.arange(0, Nz)
-    out[:, lo] += F[lo, lo - 1][None, :] * x[:, lo - 1]
-    out[:, hi] += F[hi, hi + 1][None, :] * x[:, hi + 1]
+    out[:, 1:Nz] += F[lo, lo - 1][None, :] * x[:, 0:Nz - 1]
+    out[:, 0:Nz] += F[hi, hi + 1][None, :] * x[:, 1:Nz + 1]
     out[:, 0] += F[0, Nz + 1] * x[:, Nz + 1]
     return out
 
@@ -329,13 +332,15 @@ def dp_solve_no_wall(ops, Cf, eta):
     C, Ginv, AinvB = ops.C[sl], ops.Ginv[sl], ops.AinvB[sl]
     Dx, Dy, K = ops.Dx[sl], ops.Dy[sl], ops.K[sl]
     # chebCoeffDiff (DPTools.cpp:134-174), all three components
-    df = np.zeros_like(cf)
+    cfT = np.ascontiguousarray(np.moveaxis(cf, 1, 0))     # rows first
+    dfT = np.zeros_like(cfT)
     fac = (Nz - 1) * 2 / H
-    df[:, Nz - 2] = fac * cf[:, Nz - 1]
+    dfT[Nz - 2] = fac * cfT[Nz - 1]
     for k in range(2, Nz):
         fac1 = (Nz - k) * 2 / H
-        df[:, Nz - k - 1] = df[:, Nz - k + 1] + fac1 * cf[:, Nz - k]
-    df[:, 0] /= 2
+        dfT[Nz - k - 1] = dfT[Nz - k + 1] + fac1 * cfT[Nz - k]
+    dfT[0] /= 2
+    df = np.moveaxis(dfT, 0, 1)
     # assemble_p_rhs (DPTools.cpp:177-195)
     p_rhs = Dx[:, None] * cf[:, :, 0] + Dy[:, None] * cf[:, :, 1] + df[:, :, 2]
     # schurSolve_p (DPTools.cpp:236-285)

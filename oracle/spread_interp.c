@@ -165,18 +165,24 @@ int orc_locate(int nP, const double* xP, double hx, double hy, double Lx, double
     if (ir == Nzeff) ir -= 1;
     else if (z - thresh > zext[ir]) ir -= 1;
     indl[i] = il; indr[i] = ir;
-    /* x/y support with clamp to +thresh; ParticleList.cpp:493-512 */
+    /* x/y support with clamp to +thresh; ParticleList.cpp:493-512.  The
+       reference is built with -O3 -march=native (source/cpu/CMakeLists.txt:63),
+       where GCC contracts (index * h - x) into one fused multiply-add.  The
+       offset is rounded once, so fma() here.  It matters at the support edge:
+       phi has a square-root singularity at |r| = alpha, and a one-ulp change
+       of an offset landing there moves phi by ~1e-7 relative (positions on
+       grid nodes, e.g. the reference's own x = L/2 mobility runs). */
     int xc = (int)(x / hx), yc = (int)(y / hy);
     xc += ((wx % 2) && (x / hx - xc > 1.0 / 2.0) ? 1 : 0);
     yc += ((wy % 2) && (y / hy - yc > 1.0 / 2.0) ? 1 : 0);
     xclose[i] = xc; yclose[i] = yc;
     for (int j = 0; j < wx; ++j) {
-      double v = ((double)xc + j - wx / 2 + evenx) * hx - x;
+      double v = fma((double)xc + j - wx / 2 + evenx, hx, -x);
       if (v > thresh || v < -thresh) v = thresh;
       xunwrap[j + i * wx] = v;
     }
     for (int j = 0; j < wy; ++j) {
-      double v = ((double)yc + j - wy / 2 + eveny) * hy - y;
+      double v = fma((double)yc + j - wy / 2 + eveny, hy, -y);
       if (thresh < v || v < -thresh) v = thresh;
       yunwrap[j + i * wy] = v;
     }
@@ -498,6 +504,21 @@ static void interp_impl(int kind, int posmode, double thresh, int nP, const doub
         }
     for (int d = 0; d < 3; ++d) uP[3 * p + d] = acc[d];
   }
+}
+
+/* off[i*w + j] = clamp(fma(close[i] + j - w/2 + even, h, -x[i])): the
+   uniform-direction offsets of ParticleList.cpp:339-368 (locateOnGridUnifZ),
+   rounded once as the reference build computes them (see orc_locate). */
+void orc_fma_offsets(int n, int w, const double* x, const long long* close, double h, double thresh,
+                     double* off)
+{
+  const int even = -1 * (w % 2) + 1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < w; ++j) {
+      double v = fma((double)close[i] + j - w / 2 + even, h, -x[i]);
+      if (v > thresh || v < -thresh) v = thresh;
+      off[i * w + j] = v;
+    }
 }
 
 int orc_num_threads(void)

@@ -24,9 +24,11 @@ minZ from y instead of z (ParticleList.cpp:297); positions here are shifted by
 zmin in z, which is what the reference intends and what it computes when
 zmin = 0.
 """
+import ctypes
+
 import numpy as np
 
-from oracle.dp_oracle import KernelSpec
+from oracle.dp_oracle import KernelSpec, _p, lib
 
 
 def _wrap_offsets(xs, h, N, w, thresh):
@@ -38,8 +40,13 @@ def _wrap_offsets(xs, h, N, w, thresh):
         close += ((xs / h - close) > 0.5).astype(np.int64)
     j = np.arange(w)
     idx = close[:, None] + j[None, :] - w // 2 + even
-    off = idx * h - xs[:, None]
-    off = np.where((off > thresh) | (off < -thresh), thresh, off)
+    # index * h - x rounded once, as the reference build computes it (fused
+    # multiply-add; see orc_locate in oracle/spread_interp.c), then clamped
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    close = np.ascontiguousarray(close, dtype=np.int64)
+    off = np.zeros((xs.shape[0], w))
+    lib().orc_fma_offsets(xs.shape[0], w, _p(xs),
+                          close.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), h, thresh, _p(off))
     return np.mod(idx, N), off
 
 

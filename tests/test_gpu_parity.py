@@ -32,6 +32,10 @@ CASES = [
     ("nowall_noT", "nowall", 12.0, 16.0, 9.0, 24, 32, 20, 30, False),
     ("slit_nz96", "slit", 16.0, 16.0, 10.0, 32, 32, 96, 48, True),      # 16 lanes x 6 rows
     ("bottom_nz150", "bottom", 12.0, 12.0, 9.0, 24, 24, 150, 30, True),  # 32 lanes x 8 rows
+    ("bottom_nz120", "bottom", 12.0, 12.0, 9.0, 24, 24, 120, 30, True),  # 16 lanes x 8 rows
+    # 64 lanes x 8 rows; a tall box (the reference's Nz rule gives ~300 for
+    # H = 48, h = 0.5), so that wall supports stay ~24 nodes
+    ("slit_nz300", "slit", 12.0, 12.0, 96.0, 24, 24, 300, 30, True),
 ]
 
 
