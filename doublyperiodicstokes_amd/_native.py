@@ -26,9 +26,10 @@ DPS_SOLVER_AUTO = 0
 DPS_SOLVER_LANE = 1
 DPS_F32 = 0
 DPS_F64 = 1
+DPS_NOALLOC_SCRATCH_BYTES = 268435456
 
 EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions", "dps_mdot",
-           "dps_mdot_adjoint", "dps_batch_create", "dps_batch_destroy", "dps_batch_mdot", "dps_batch_tune",
+           "dps_mdot_adjoint", "dps_mdot_adjoint_noalloc", "dps_batch_create", "dps_batch_destroy", "dps_batch_mdot", "dps_batch_tune",
            "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
            "dps_get_timings", "dps_get_precision", "dps_last_error")
@@ -73,6 +74,7 @@ def load():
     L.dps_update_positions.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
     L.dps_mdot.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_mdot_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
+    L.dps_mdot_adjoint_noalloc.argtypes = L.dps_mdot_adjoint.argtypes
     pp = ctypes.POINTER(vp)
     L.dps_batch_create.argtypes = [pp, ctypes.c_int, ctypes.c_int, pp]
     L.dps_batch_destroy.argtypes = [vp]

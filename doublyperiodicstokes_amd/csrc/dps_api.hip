@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/dpstokes_hip.h"
@@ -49,8 +50,49 @@ int g_rocfft_users = 0;
 
 }  // namespace
 
+// Scratch of dps_mdot_adjoint_noalloc: the adjoint's second velocity grid and
+// its derivative records, in one array per GPU that lives as long as the
+// process.  Host code never uses the array's name as an address: each device's
+// copy is reached through the pointer hipGetSymbolAddress gave on that device
+// (g_scratch_ptr), which is what kernels receive.  Its use by host code
+// (hipGetSymbolAddress) makes the compiler emit it with global binding in the
+// device code object, where the loader can find it; no attribute may force it
+// to stay local.
+static __device__ __align__(256) unsigned char scratch[DPS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current (called by dps_create).
+int scratch_init(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return fail(DPS_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return DPS_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(DPS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = (unsigned char*)p;
+  return DPS_OK;
+}
+
+unsigned char* scratch_of(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
 struct dps_handle {
   dps_params P;
+  int device = 0;   // HIP device current at dps_create
   int N = 0;
   int geometry = 0, prec = 0;
   double hx = 0, hy = 0, Lz = 0, H = 0, hz = 0;   // hz: periodic z spacing
@@ -1002,21 +1044,39 @@ int do_interp(dps_handle* h, void* vel, void* ang, int dtype, hipStream_t s)
   return DPS_OK;
 }
 
+// record lengths of the position adjoint's force-kernel derivative records
+// and torque-kernel second-derivative records
+void adjoint_record_sizes(dps_handle* h)
+{
+  h->rs_md = 2 * h->km.wx + 2 * h->km.wy + 4 * h->km.wzmax;
+  h->rs_d2 = (h->has_dip && !h->reg) ? 4 * h->kd.wx + 4 * h->kd.wy + 4 * h->kd.wzmax : 0;
+}
+
+// bytes of the position adjoint's buffers: support records, derivative
+// records, z ranges, second velocity grid, torque second-derivative records
+void adjoint_buffer_bytes(const dps_handle* h, size_t bytes[5])
+{
+  const size_t ng = (size_t)3 * h->Nz * h->Ny * h->Nx;
+  bytes[0] = (size_t)h->N * sizeof(int4);
+  bytes[1] = (size_t)h->N * h->rs_md * h->tsz;
+  bytes[2] = (size_t)h->N * sizeof(int4);
+  bytes[3] = ng * h->tsz;
+  bytes[4] = (size_t)h->N * h->rs_d2 * h->tsz;
+}
+
 // buffers of the position adjoint, allocated on its first call
 int ensure_adjoint(dps_handle* h)
 {
   if (h->d_gridA) return DPS_OK;
   int rc;
-  const size_t ng = (size_t)3 * h->Nz * h->Ny * h->Nx;
-  h->rs_md = 2 * h->km.wx + 2 * h->km.wy + 4 * h->km.wzmax;
-  if ((rc = h->alloc(&h->d_sup_md, h->N))) return rc;
-  if ((rc = h->alloc((unsigned char**)&h->d_rec_md, (size_t)h->N * h->rs_md * h->tsz))) return rc;
-  if ((rc = h->alloc(&h->d_zrange_md, h->N))) return rc;
-  if ((rc = h->alloc((unsigned char**)&h->d_gridA, ng * h->tsz))) return rc;
-  if (h->has_dip && !h->reg) {
-    h->rs_d2 = 4 * h->kd.wx + 4 * h->kd.wy + 4 * h->kd.wzmax;
-    if ((rc = h->alloc((unsigned char**)&h->d_rec_d2, (size_t)h->N * h->rs_d2 * h->tsz))) return rc;
-  }
+  adjoint_record_sizes(h);
+  size_t b[5];
+  adjoint_buffer_bytes(h, b);
+  if ((rc = h->alloc((unsigned char**)&h->d_sup_md, b[0]))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_rec_md, b[1]))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_zrange_md, b[2]))) return rc;
+  if ((rc = h->alloc((unsigned char**)&h->d_gridA, b[3]))) return rc;
+  if (h->rs_d2 > 0 && (rc = h->alloc((unsigned char**)&h->d_rec_d2, b[4]))) return rc;
   return DPS_OK;
 }
 
@@ -1060,6 +1120,95 @@ int check_handle(const dps_handle* h, bool need_pos)
   return DPS_OK;
 }
 
+// argument rules shared by the two adjoint entry points
+int adjoint_check(dps_handle* h, const void* forces, const void* torques, const void* grad_velocities,
+                  const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
+                  int dtype)
+{
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (h->N > 0 && !grad_velocities) return fail(DPS_ERR_ARG, "null grad_velocities");
+  if ((grad_angular || grad_torques || torques) && !h->has_dip)
+    return fail(DPS_ERR_ARG, "torque terms given but no dipole kernel (w_d = 0)");
+  if (grad_torques && !grad_forces) return fail(DPS_ERR_ARG, "grad_torques needs grad_forces");
+  if (!grad_positions) {
+    if (grad_torques && !grad_angular)
+      return fail(DPS_ERR_ARG, "grad_torques without positions needs grad_angular (zeros if W is unused)");
+    return DPS_OK;
+  }
+  if ((torques || grad_angular || grad_torques) && h->reg)
+    return fail(DPS_ERR_ARG, "position gradients with torques need the derivative torque kernels");
+  if (h->N > 0 && !forces) return fail(DPS_ERR_ARG, "null forces");
+  return DPS_OK;
+}
+
+// the position adjoint on the handle's adjoint buffers (d_sup_md, d_rec_md,
+// d_zrange_md, d_gridA, d_rec_d2), which the caller has pointed at memory
+int adjoint_positions(dps_handle* h, const void* forces, const void* torques, const void* grad_velocities,
+                      const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
+                      int dtype, hipStream_t s)
+{
+  int rc = DPS_OK;
+  const bool tterms = torques || grad_angular;
+  const bool p = h->prof;
+  h->prof = false;
+  rc = locate_force_derivatives(h, s);
+  // u = velocity grid of (F, T) into d_gridA, v = that of (gU, gW) into d_grid
+  if (!rc) rc = do_spread(h, forces, torques, dtype, s);
+  if (!rc) rc = do_solve(h, s, 1);
+  std::swap(h->d_grid, h->d_gridA);
+  if (!rc) rc = do_spread(h, grad_velocities, grad_angular, dtype, s);
+  if (!rc) rc = do_solve(h, s, 1);
+  if (!rc && grad_forces) rc = do_interp(h, grad_forces, grad_torques, dtype, s);
+  if (!rc) {
+    const InterpArgs A = interp_args(h);
+    const KernelDesc K = force_deriv_kernel(h);
+    if (mixed(h)) {
+      if (dtype == DPS_F32)
+        launch_interp_grad<float, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                         grad_velocities, forces, grad_positions, s);
+      else
+        launch_interp_grad<float, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                          grad_velocities, forces, grad_positions, s);
+    } else {
+      if (dtype == DPS_F32)
+        launch_interp_grad<double, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                          grad_velocities, forces, grad_positions, s);
+      else
+        launch_interp_grad<double, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
+                                           grad_velocities, forces, grad_positions, s);
+    }
+    if (tterms) {
+      const StencilArgs S = stencil_args(h);
+      const void* mu = grad_angular;
+      const void* tq = torques;
+      if (mixed(h)) {
+        launch_stencil_d2<float>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
+        if (dtype == DPS_F32)
+          launch_interp_grad_t<float, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                             mu, tq, grad_positions, s);
+        else
+          launch_interp_grad_t<float, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                              mu, tq, grad_positions, s);
+      } else {
+        launch_stencil_d2<double>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
+        if (dtype == DPS_F32)
+          launch_interp_grad_t<double, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
+                                              mu, tq, grad_positions, s);
+        else
+          launch_interp_grad_t<double, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA,
+                                               h->d_grid, mu, tq, grad_positions, s);
+      }
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = fail(DPS_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  }
+  std::swap(h->d_grid, h->d_gridA);
+  h->prof = p;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1086,6 +1235,8 @@ int dps_create(const dps_params* params, int number_particles, dps_handle** out)
   h->P = P;
   h->N = number_particles;
   int rc = build(h);
+  if (!rc) rc = hipGetDevice(&h->device) == hipSuccess ? DPS_OK : fail(DPS_ERR_HIP, "hipGetDevice failed");
+  if (!rc) rc = scratch_init(h->device);
   bool fft_user = false;
   if (!rc) {
     if (g_rocfft_users++ == 0) rocfft_setup();
@@ -1188,81 +1339,62 @@ int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, con
                      const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
                      int dtype, void* stream)
 {
-  int rc = check_handle(h, true);
+  int rc = adjoint_check(h, forces, torques, grad_velocities, grad_angular, grad_forces, grad_torques,
+                         grad_positions, dtype);
   if (rc) return rc;
-  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
-  if (h->N > 0 && !grad_velocities) return fail(DPS_ERR_ARG, "null grad_velocities");
-  if ((grad_angular || grad_torques || torques) && !h->has_dip)
-    return fail(DPS_ERR_ARG, "torque terms given but no dipole kernel (w_d = 0)");
-  if (grad_torques && !grad_forces) return fail(DPS_ERR_ARG, "grad_torques needs grad_forces");
   if (!grad_positions) {
-    if (grad_torques && !grad_angular)
-      return fail(DPS_ERR_ARG, "grad_torques without positions needs grad_angular (zeros if W is unused)");
     // the solve is taken as self-adjoint: [dL/dF; dL/dT] = M [gU; gW]
     if (!grad_forces) return DPS_OK;
     return dps_mdot(h, grad_velocities, grad_angular, grad_forces, grad_torques, dtype, stream);
   }
-  const bool tterms = torques || grad_angular;
-  if ((tterms || grad_torques) && h->reg)
-    return fail(DPS_ERR_ARG, "position gradients with torques need the derivative torque kernels");
-  if (h->N > 0 && !forces) return fail(DPS_ERR_ARG, "null forces");
   if ((rc = ensure_adjoint(h))) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  const bool p = h->prof;
-  h->prof = false;
-  rc = locate_force_derivatives(h, s);
-  // u = velocity grid of (F, T) into d_gridA, v = that of (gU, gW) into d_grid
-  if (!rc) rc = do_spread(h, forces, torques, dtype, s);
-  if (!rc) rc = do_solve(h, s, 1);
-  std::swap(h->d_grid, h->d_gridA);
-  if (!rc) rc = do_spread(h, grad_velocities, grad_angular, dtype, s);
-  if (!rc) rc = do_solve(h, s, 1);
-  if (!rc && grad_forces) rc = do_interp(h, grad_forces, grad_torques, dtype, s);
-  if (!rc) {
-    const InterpArgs A = interp_args(h);
-    const KernelDesc K = force_deriv_kernel(h);
-    if (mixed(h)) {
-      if (dtype == DPS_F32)
-        launch_interp_grad<float, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
-                                         grad_velocities, forces, grad_positions, s);
-      else
-        launch_interp_grad<float, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
-                                          grad_velocities, forces, grad_positions, s);
-    } else {
-      if (dtype == DPS_F32)
-        launch_interp_grad<double, float>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
-                                          grad_velocities, forces, grad_positions, s);
-      else
-        launch_interp_grad<double, double>(A, h->d_sup_md, h->d_rec_md, h->rs_md, K, h->d_gridA, h->d_grid,
-                                           grad_velocities, forces, grad_positions, s);
-    }
-    if (tterms) {
-      const StencilArgs S = stencil_args(h);
-      const void* mu = grad_angular;
-      const void* tq = torques;
-      if (mixed(h)) {
-        launch_stencil_d2<float>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
-        if (dtype == DPS_F32)
-          launch_interp_grad_t<float, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
-                                             mu, tq, grad_positions, s);
-        else
-          launch_interp_grad_t<float, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
-                                              mu, tq, grad_positions, s);
-      } else {
-        launch_stencil_d2<double>(S, h->d_sup_d, h->d_rec_d2, h->rs_d2, s);
-        if (dtype == DPS_F32)
-          launch_interp_grad_t<double, float>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA, h->d_grid,
-                                              mu, tq, grad_positions, s);
-        else
-          launch_interp_grad_t<double, double>(A, h->d_sup_d, h->d_rec_d2, h->rs_d2, h->kd, h->d_gridA,
-                                               h->d_grid, mu, tq, grad_positions, s);
-      }
-    }
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = fail(DPS_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  return adjoint_positions(h, forces, torques, grad_velocities, grad_angular, grad_forces, grad_torques,
+                           grad_positions, dtype, (hipStream_t)stream);
+}
+
+int dps_mdot_adjoint_noalloc(dps_handle* h, const void* forces, const void* torques,
+                             const void* grad_velocities, const void* grad_angular, void* grad_forces,
+                             void* grad_torques, void* grad_positions, int dtype, void* stream)
+{
+  int rc = adjoint_check(h, forces, torques, grad_velocities, grad_angular, grad_forces, grad_torques,
+                         grad_positions, dtype);
+  if (rc) return rc;
+  if (!grad_positions) {
+    if (!grad_forces) return DPS_OK;
+    return dps_mdot(h, grad_velocities, grad_angular, grad_forces, grad_torques, dtype, stream);
   }
-  std::swap(h->d_grid, h->d_gridA);
-  h->prof = p;
+  adjoint_record_sizes(h);
+  size_t b[5], off[5], need = 0;
+  adjoint_buffer_bytes(h, b);
+  for (int i = 0; i < 5; ++i) {
+    off[i] = need;
+    need += (b[i] + 255) / 256 * 256;
+  }
+  if (need > (size_t)DPS_NOALLOC_SCRATCH_BYTES)
+    return fail(DPS_ERR_ARG, "the position adjoint of this solver needs " + std::to_string(need) +
+                                 " bytes of scratch, more than the " +
+                                 std::to_string((size_t)DPS_NOALLOC_SCRATCH_BYTES) +
+                                 " bytes of dps_mdot_adjoint_noalloc's scratch; use dps_mdot_adjoint");
+  unsigned char* base = scratch_of(h->device);
+  if (!base) return fail(DPS_ERR_STATE, "the no-allocation scratch is not set up on this device");
+  // point the adjoint's buffers at the scratch for this call, then restore
+  int4* sup_md = h->d_sup_md;
+  void* rec_md = h->d_rec_md;
+  int4* zrange_md = h->d_zrange_md;
+  void* gridA = h->d_gridA;
+  void* rec_d2 = h->d_rec_d2;
+  h->d_sup_md = (int4*)(base + off[0]);
+  h->d_rec_md = base + off[1];
+  h->d_zrange_md = (int4*)(base + off[2]);
+  h->d_gridA = base + off[3];
+  h->d_rec_d2 = h->rs_d2 > 0 ? (void*)(base + off[4]) : nullptr;
+  rc = adjoint_positions(h, forces, torques, grad_velocities, grad_angular, grad_forces, grad_torques,
+                         grad_positions, dtype, (hipStream_t)stream);
+  h->d_sup_md = sup_md;
+  h->d_rec_md = rec_md;
+  h->d_zrange_md = zrange_md;
+  h->d_gridA = gridA;
+  h->d_rec_d2 = rec_d2;
   return rc;
 }
 

@@ -148,6 +148,26 @@ int dps_mdot_adjoint(dps_handle* h, const void* forces, const void* torques, con
                      const void* grad_angular, void* grad_forces, void* grad_torques, void* grad_positions,
                      int dtype, void* stream);
 
+/* dps_mdot_adjoint without any device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc),
+   for which the first position-gradient call of dps_mdot_adjoint is unusable
+   because it allocates the adjoint's second velocity grid and derivative
+   records.  Same arguments, results and errors as dps_mdot_adjoint; those
+   buffers come instead from one static scratch array per GPU of
+   DPS_NOALLOC_SCRATCH_BYTES bytes, set up by dps_create and held until the
+   process ends.  Solvers whose adjoint needs more (second grid 3 nx ny nz
+   reals plus records of a few hundred reals per particle: the 256^2 x 96
+   slit workload in FP64 fits, the 1024^2 x 48 monolayer does not) get
+   DPS_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that
+   overlap on one GPU (other streams, other threads) race, and must be
+   ordered by the caller.  The array keeps the last call's contents; nothing
+   in it needs to start at zero.  The handle's device must be current. */
+#define DPS_NOALLOC_SCRATCH_BYTES 268435456
+int dps_mdot_adjoint_noalloc(dps_handle* h, const void* forces, const void* torques,
+                             const void* grad_velocities, const void* grad_angular, void* grad_forces,
+                             void* grad_torques, void* grad_positions, int dtype, void* stream);
+
 /* Stage entry points (for stage-level parity tests and the adjoint):
    spread forces (+torque curl) to the real grid, returned in the reference's
    interior layout (Ny, Nx, Nz, 3) FP64; solve grid -> velocity grid (same
