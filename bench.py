@@ -131,7 +131,12 @@ def copy_probe(torch, mb):
 
 
 def cpu_baseline(wl, pos, F, T, budget_s, reg=False):
-    """FP64 oracle Mdot timed on the host: as many full calls as fit in the budget (at least one)."""
+    """FP64 oracle Mdot timed on the host, as the reference's timing harness
+    does it (source/cpu/examples/mobility_bench.py:312-442): one untimed
+    warm-up call, then timed calls until `budget_s` seconds or 20 calls, with
+    the harness's phases (SI force, SI torque, FCT+iFCT, Solve).  Returns the
+    rate, the call count, the timed seconds, the thread count, the mean
+    seconds per phase and the oracle's (U, W) of the last call."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import dp_oracle as o
     geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
@@ -140,15 +145,20 @@ def cpu_baseline(wl, pos, F, T, budget_s, reg=False):
                            has_torque=torque, w_d=6 if torque else None,
                            betaw_d=1.785 * 6 if torque else None, reg_kernel=reg)
     ref.set_positions(pos)
+    call = (lambda: ref.mdot(F, T) + ({},)) if reg else (lambda: ref.mdot_phases(F, T))
+    call()
     n = 0
+    phases = {}
     t0 = time.perf_counter()
     while True:
-        ref.mdot(F, T)
+        U, W, ph = call()
+        for k, v in ph.items():
+            phases[k] = phases.get(k, 0.0) + v
         n += 1
         el = time.perf_counter() - t0
         if el > budget_s or n >= 20:
             break
-    return n / el, n, el, o.num_threads()
+    return n / el, n, el, o.num_threads(), {k: v / n for k, v in phases.items()}, U, W
 
 
 def batch_workload(n_inst=64, seed=0):
@@ -292,8 +302,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="slit16k", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=12.0,
-                    help="seconds of CPU-baseline Mdot work (0 disables)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of timed CPU-baseline Mdot work after one warm-up call (0 disables)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--reg-kernel", action="store_true",
                     help="torques through the regular kernel with spectral curls (useRegKernel)")
@@ -379,6 +389,7 @@ def main():
                      "launch_us": stage_us[dom]},
     }
     del mixed
+    U64 = W64 = None
     if not args.no_fp64:
         f64 = gpu_solver(wl, "double", reg, args.solver)
         f64.setPositions(pos)
@@ -397,13 +408,24 @@ def main():
     if args.copy_probe_mb > 0:
         copy_probe(torch, args.copy_probe_mb)
     if args.cpu_budget > 0:
-        rate, n, el, thr = cpu_baseline(wl, pos, F, T, args.cpu_budget, reg)
+        rate, n, el, thr, phases, U_ref, W_ref = cpu_baseline(wl, pos, F, T, args.cpu_budget, reg)
         out["cpu_baseline"] = {"value": rate, "unit": "Mdot/s", "cores": thr, "kind": "port",
-                               "sample": "%d full-size FP64 oracle Mdot call(s) on the same inputs (%.1f s)"
-                               % (n, el)}
+                               "sample": "1 warm-up + %d timed full-size FP64 oracle Mdot calls on the same "
+                                         "inputs (%.1f s)" % (n, el),
+                               "phase_ms": {k: 1000.0 * v for k, v in phases.items()}}
         out["speedup_vs_cpu"] = out["value"] / rate
         if "fp64_ms_per_step" in out:
             out["fp64_speedup_vs_cpu"] = (1000.0 / out["fp64_ms_per_step"]) / rate
+            out["fp64_over_mixed_time"] = out["fp64_ms_per_step"] / out["ms_per_step"]
+        # parity of the timed GPU paths against the oracle on the same inputs
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        out["rel_l2_U_mixed_vs_oracle"] = rel(U32.double().cpu().numpy(), U_ref)
+        if torque:
+            out["rel_l2_W_mixed_vs_oracle"] = rel(W32.double().cpu().numpy(), W_ref)
+        if U64 is not None:
+            out["rel_l2_U_fp64_vs_oracle"] = rel(U64.cpu().numpy(), U_ref)
+            if torque:
+                out["rel_l2_W_fp64_vs_oracle"] = rel(W64.cpu().numpy(), W_ref)
     print(json.dumps(out), flush=True)
 
 

@@ -28,7 +28,8 @@ DPS_F32 = 0
 DPS_F64 = 1
 DPS_NOALLOC_SCRATCH_BYTES = 268435456
 
-EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions", "dps_mdot",
+EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions",
+           "dps_check_positions", "dps_mdot",
            "dps_mdot_adjoint", "dps_mdot_adjoint_noalloc", "dps_batch_create", "dps_batch_destroy", "dps_batch_mdot", "dps_batch_tune",
            "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
@@ -72,6 +73,7 @@ def load():
     L.dps_destroy.argtypes = [vp]
     L.dps_set_positions.argtypes = [vp, vp, ctypes.c_int, vp]
     L.dps_update_positions.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
+    L.dps_check_positions.argtypes = [vp, vp]
     L.dps_mdot.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_mdot_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.dps_mdot_adjoint_noalloc.argtypes = L.dps_mdot_adjoint.argtypes

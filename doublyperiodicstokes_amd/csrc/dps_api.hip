@@ -824,6 +824,27 @@ void spread_pass(const SpreadArgs& A, const void* F, const void* Tq, bool mx, in
 // forces (and the torque curl of the derivative kernels) into d_grid; with
 // the regular torque kernel the torques go, as a plain vector field, into
 // d_gridT
+// whether every spreading pass do_spread runs (with or without torques) fits
+// the LDS budget with z chunks of zc levels
+bool spread_fits(dps_handle* h, bool torques, int zc)
+{
+  SpreadArgs passes[2];
+  int n = 0;
+  if (h->reg) {
+    passes[n++] = spread_args(h);
+    if (torques) passes[n++] = spread_args_torque(h);
+  } else if (torques && spread_pairable(h)) {
+    passes[n++] = spread_args_pair(h);
+  } else {
+    passes[n++] = spread_args(h);
+  }
+  for (int i = 0; i < n; ++i) {
+    passes[i].ZC = zc;
+    if (spread_npf(passes[i].rs_max) == 0 || spread_lds_bytes(passes[i], h->tsz) > 65536) return false;
+  }
+  return true;
+}
+
 int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_t s)
 {
   const void* Tq = (T && h->has_dip) ? T : nullptr;
@@ -1308,7 +1329,39 @@ int dps_update_positions(dps_handle* h, void* positions, const void* velocities,
   if (dtype == DPS_F32) launch_advance<float>(positions, velocities, dt, h->N, 0.0, Lx, 0.0, Ly, s);
   else launch_advance<double>(positions, velocities, dt, h->N, 0.0, Lx, 0.0, Ly, s);
   HIPCHK(hipGetLastError());
-  return dps_set_positions(h, positions, dtype, stream);
+  // without valid positions (first call, or after a reported range error)
+  // this is a set: synchronous, with a fresh error flag
+  if (!h->positions_set) return dps_set_positions(h, positions, dtype, stream);
+  // locate as dps_set_positions does, without its synchronisation: a z-range
+  // error stays set on the device until dps_check_positions reads it
+  const StencilArgs A = stencil_args(h);
+  if (mixed(h)) {
+    if (dtype == DPS_F32) launch_stencil<float, float>(A, positions, s);
+    else launch_stencil<double, float>(A, positions, s);
+  } else {
+    if (dtype == DPS_F32) launch_stencil<float, double>(A, positions, s);
+    else launch_stencil<double, double>(A, positions, s);
+  }
+  HIPCHK(hipGetLastError());
+  launch_sort(h->d_key, h->d_counts, h->d_rank, h->d_start, h->d_order, h->N, h->ntiles, s);
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
+}
+
+int dps_check_positions(dps_handle* h, void* stream)
+{
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  launch_reset_err(h->d_err, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  if (*h->h_err) {
+    h->positions_set = false;
+    return fail(DPS_ERR_RANGE, "Position is outside of z extent");
+  }
+  return DPS_OK;
 }
 
 int dps_mdot(dps_handle* h, const void* forces, const void* torques, void* velocities, void* angular,
@@ -1405,6 +1458,9 @@ int dps_stage_spread(dps_handle* h, const void* forces, const void* torques, int
   if (rc) return rc;
   if (torques && h->reg)
     return fail(DPS_ERR_ARG, "with the regular torque kernel the torque curl is spectral: no real-space torque grid");
+  if (dtype != DPS_F32 && dtype != DPS_F64) return fail(DPS_ERR_ARG, "dtype must be DPS_F32 or DPS_F64");
+  if (h->N > 0 && (!forces || !grid_out)) return fail(DPS_ERR_ARG, "null forces or grid");
+  if (torques && !h->has_dip) return fail(DPS_ERR_ARG, "torques given but no dipole kernel (w_d = 0)");
   hipStream_t s = (hipStream_t)stream;
   if ((rc = do_spread(h, forces, torques, dtype, s))) return rc;
   if (mixed(h)) launch_grid_to_ref<float>(h->d_grid, grid_out, h->Nx, h->Ny, h->Nz, s);
@@ -1570,16 +1626,16 @@ int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* t
   for (size_t i = 0; i < b->hs.size() && !rc; ++i) {
     dps_handle* h = b->hs[i];
     if ((rc = check_handle(h, true))) break;
+    if (h->N > 0 && !forces[i]) { rc = fail(DPS_ERR_ARG, "null forces"); break; }
+    const void* t = torques ? torques[i] : nullptr;
+    if (t && !h->has_dip) { rc = fail(DPS_ERR_ARG, "torques given but no dipole kernel (w_d = 0)"); break; }
     const int keep = h->ZC;
     int best = keep;
     float best_ms = 1e30f;
     for (int zc : cands) {
-      if (zc > h->Nz) continue;
-      SpreadArgs S = spread_args(h);
-      S.ZC = zc;
-      if (spread_lds_bytes(S, h->tsz) > 65536) continue;
+      // the candidate must fit the passes do_spread will run for these inputs
+      if (zc > h->Nz || !spread_fits(h, t != nullptr, zc)) continue;
       h->ZC = zc;
-      const void* t = (torques && h->has_dip) ? torques[i] : nullptr;
       // one untimed call, then the spreading of three calls timed together
       if ((rc = do_spread(h, forces[i], t, dtype, s))) break;
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(DPS_ERR_HIP, "event record failed"); break; }

@@ -192,11 +192,15 @@ class DPStokes:
         N.check(N.load().dps_set_positions(self._h, ctypes.c_void_p(p.data_ptr()),
                                            _dtype_code(p.dtype), _stream()))
 
-    def update_positions_device(self, positions, velocities=None, dt=0.0):
+    def update_positions_device(self, positions, velocities=None, dt=0.0, check=True):
         """One position update kept on the device (ParticleList::update,
         ParticleList.cpp:550-753): ``positions += dt * velocities`` in place on a
         device tensor, x and y rewrapped into the unit cell, then relocated as
-        by setPositions.  ``velocities`` must have the positions' dtype."""
+        by setPositions.  ``velocities`` must have the positions' dtype.  With
+        ``check`` (the default) the stream is synchronised and a particle
+        that left the z extent raises ValueError, as the reference exits;
+        ``check=False`` leaves the host unsynchronised (step loops, graph
+        capture) and ``check_positions()`` reports later."""
         self._need()
         torch = _torch()
         if not (isinstance(positions, torch.Tensor) and positions.is_cuda and positions.is_contiguous()):
@@ -211,6 +215,14 @@ class DPStokes:
             v = ctypes.c_void_p(velocities.data_ptr())
         N.check(N.load().dps_update_positions(self._h, ctypes.c_void_p(positions.data_ptr()), v,
                                               float(dt), _dtype_code(positions.dtype), _stream()))
+        if check:
+            self.check_positions()
+
+    def check_positions(self):
+        """Synchronise and raise ValueError if a position update since the last
+        check moved a particle out of the z extent (dps_check_positions)."""
+        self._need()
+        N.check(N.load().dps_check_positions(self._h, _stream()))
 
     def mdot_device(self, forces, torques=None, out_dtype=None):
         """Mdot on device tensors; returns (velocities, angular velocities or None)."""

@@ -112,9 +112,23 @@ int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* str
    caller's array too (ParticleList.cpp:556-558) -- then the particles are
    located as by dps_set_positions.  A caller's integrator can loop Mdot and
    this call with the positions never leaving the device.  Both arrays use
-   `dtype`; DPS_ERR_RANGE if a particle leaves the z extent. */
+   `dtype`.  Unlike dps_set_positions it does not synchronise the stream, so
+   a step loop of dps_mdot and this call can be captured in a HIP graph: a
+   particle leaving the z extent (the reference exits) sets an error flag on
+   the device, which stays set over further updates until dps_check_positions
+   reports it; until then such a particle is located at the nearest end of
+   the z extent.  Without valid positions (before dps_set_positions, or after
+   a reported range error) the call synchronises and reports DPS_ERR_RANGE
+   itself, as dps_set_positions does. */
 int dps_update_positions(dps_handle* h, void* positions, const void* velocities, double dt, int dtype,
                          void* stream);
+
+/* Synchronise `stream` and report whether a dps_update_positions since the
+   last check (or since dps_set_positions) moved a particle out of the z
+   extent: DPS_ERR_RANGE (and the positions count as unset, as after a
+   failed dps_set_positions), else DPS_OK.  Clears the flag.  No counterpart
+   in the reference, whose update exits at once (ParticleList.cpp:595-600). */
+int dps_check_positions(dps_handle* h, void* stream);
 
 /* Mobility product (DPStokesGlue::Mdot, uammd_wrapper.cu:146-170; FCM.Mdot,
    FCM.py:334-441): velocities = M_tt F + M_tr T, angular = M_rt F + M_rr T.

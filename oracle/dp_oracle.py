@@ -735,6 +735,69 @@ class OracleDPStokes:
         om[2::3] = -1 / 2 * (dxv[1::3] - dyv[0::3])
         return v, om
 
+    def mdot_phases(self, forces, torques=None):
+        """mdot (derivative torque kernels) with the phase timers of the
+        reference's timing harness, source/cpu/examples/mobility_bench.py:312-442:
+        'SI force' (force spread + velocity interpolation), 'SI torque' (the
+        three derivative spreads with the curl + the derivative interpolations
+        and Omega), 'FCT+iFCT' (forward and backward Fourier-Chebyshev
+        transforms), 'Solve' (per-mode BVP, wall correction, k = 0).  Returns
+        (U, W, seconds per phase); the arithmetic is that of mdot."""
+        import time
+        if self.reg_kernel:
+            raise ValueError('phase timing covers the derivative torque kernels only')
+        t = {'SI force': 0.0, 'SI torque': 0.0, 'FCT+iFCT': 0.0, 'Solve': 0.0}
+        c = time.perf_counter
+        torque = self.has_torque and torques is not None and np.size(torques) > 0
+        t0 = c()
+        self._ext_f = self._ext_buffer(self._loc)
+        F = self._spread(0, self._loc, self.mono, forces, self.BCs, self._ext_f)
+        t['SI force'] += c() - t0
+        if torque:
+            t0 = c()
+            self._ext_t = self._ext_buffer(self._tloc)
+            dxT = self._spread(1, self._tloc, self.dip, torques, self.BCs, self._ext_t)
+            F[..., 1] -= 0.5 * dxT[..., 2]
+            F[..., 2] += 0.5 * dxT[..., 1]
+            dyT = self._spread(2, self._tloc, self.dip, torques, self.BCs, self._ext_t)
+            F[..., 0] += 0.5 * dyT[..., 2]
+            F[..., 2] -= 0.5 * dyT[..., 0]
+            dzT = self._spread(3, self._tloc, self.dip, torques, self.BCst_z, self._ext_t)
+            F[..., 0] -= 0.5 * dzT[..., 1]
+            F[..., 1] += 0.5 * dzT[..., 0]
+            t['SI torque'] += c() - t0
+        Ny, Nx, Nz = self.Ny, self.Nx, self.Nz
+        t0 = c()
+        Cf = fc_forward(F).reshape(Ny * Nx, Nz, 3)
+        t['FCT+iFCT'] += c() - t0
+        t0 = c()
+        U, P = dp_solve_no_wall(self.ops, Cf, self.eta)
+        if self.geometry == 'bottom':
+            bottom_wall_correction(self.ops, U, P, self.zpts, self.eta)
+            k0_wall_velocity(self.ops, Cf[0], U, self.eta, 'bottom')
+        elif self.geometry == 'slit':
+            slit_channel_correction(self.ops, U, P, self.zpts, self.Lz, self.eta)
+            k0_wall_velocity(self.ops, Cf[0].real + 1j * Cf[0].real, U, self.eta, 'slit')
+        t['Solve'] += c() - t0
+        t0 = c()
+        uG = fc_backward(U.reshape(Ny, Nx, Nz, 3))
+        t['FCT+iFCT'] += c() - t0
+        t0 = c()
+        v = self._interp(0, self._loc, self.mono, uG, self.BCs, self._ext_f)
+        t['SI force'] += c() - t0
+        om = None
+        if torque:
+            t0 = c()
+            dxv = self._interp(1, self._tloc, self.dip, uG, self.BCs, self._ext_t)
+            dyv = self._interp(2, self._tloc, self.dip, uG, self.BCs, self._ext_t)
+            dzv = self._interp(3, self._tloc, self.dip, uG, self.BCst_z, self._ext_t)
+            om = np.empty(3 * self.N)
+            om[0::3] = -1 / 2 * (dyv[2::3] - dzv[1::3])
+            om[1::3] = -1 / 2 * (dzv[0::3] - dxv[2::3])
+            om[2::3] = -1 / 2 * (dxv[1::3] - dyv[0::3])
+            t['SI torque'] += c() - t0
+        return v, om, t
+
     def _interp_pos(self, kind, loc, ks, fG, BCs):
         L = lib()
         uP = np.zeros(3 * self.N)

@@ -84,3 +84,58 @@ def test_update_wraps_in_place_and_reports_z_range():
     with pytest.raises(ValueError):
         s.update_positions_device(p_dev, vel, 0.75 * Lz)
     s.clear()
+
+
+def test_step_loop_without_host_sync_and_as_a_graph():
+    """dps_update_positions leaves the host unsynchronised, so a time step
+    (Mdot, then the update) can be captured in a HIP graph.  Five Euler steps
+    run eagerly with check=False; the same step captured once with
+    torch.cuda.graph and replayed five times, with eager work and a
+    synchronisation between replays, gives each step's positions (FP64;
+    spreading adds with float atomics, so to 1e-12 of the box).  An update
+    that leaves the z extent is reported by the next check only."""
+    import torch
+    case = CASES[1]
+    name, geom, Lx, Ly, Lz, nx, ny, nz, N, torque = case
+    say("case %s: step loop without host sync, and as a graph" % name)
+    pos, F, T = make_case(case, seed=9)
+    pos[2::3] = 0.25 * Lz + 0.5 * pos[2::3]
+    dt = 0.05
+    Fd = torch.tensor(F, dtype=torch.float64, device="cuda")
+    Td = torch.tensor(T, dtype=torch.float64, device="cuda") if T is not None else None
+    s = gpu_for(case, "double")
+    p_e = torch.tensor(pos, dtype=torch.float64, device="cuda")
+    s.setPositions(p_e)
+    traj = []
+    for _ in range(5):
+        U, _W = s.mdot_device(Fd, Td)
+        s.update_positions_device(p_e, U, dt, check=False)
+        traj.append(p_e.clone())
+    s.check_positions()
+    g_s = gpu_for(case, "double")
+    p_g = torch.tensor(pos, dtype=torch.float64, device="cuda")
+    g_s.setPositions(p_g)
+    g_s.mdot_device(Fd, Td)     # loads code objects and FFT plans outside the capture
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        Ug, _Wg = g_s.mdot_device(Fd, Td)
+        g_s.update_positions_device(p_g, Ug, dt, check=False)
+    worst = 0.0
+    for k in range(5):
+        graph.replay()
+        torch.cuda.synchronize()
+        _ = (Fd * 2.0).sum().item()          # eager work between replays
+        g_s.check_positions()
+        e = float((p_g - traj[k]).abs().max()) / max(Lx, Ly, Lz)
+        worst = max(worst, e)
+    say("  graph replays vs eager steps: max |dx| / L = %.3e" % worst)
+    assert worst <= 1e-12
+    # out of range: not reported by the update, reported by the check
+    vel = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    vel[2] = 1.0
+    s.update_positions_device(p_e, vel, 2.0 * Lz, check=False)
+    with pytest.raises(ValueError):
+        s.check_positions()
+    s.clear()
+    g_s.clear()
