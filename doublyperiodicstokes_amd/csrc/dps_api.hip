@@ -105,6 +105,7 @@ struct dps_handle {
   ChebTables cheb;
   ChebTransform ct;
   int TX = 16, TY = 4, ZC = 32, ntx = 0, nty = 0, ntiles = 0, RX = 1, RY = 1;
+  size_t lds_max = 65536;   // LDS bytes one workgroup may use on this device
   size_t tsz = 8;   // bytes per grid real
 
   std::vector<void*> allocs;
@@ -383,14 +384,22 @@ int build_tiles(dps_handle* h, int wmax)
             : h->reg ? h->kd.wx + h->kd.wy + 2 * h->kd.wzmax
                      : 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
   {
-    // the spreading workgroup's LDS must fit in 64 KB: shorten the z chunk
+    // the spreading workgroup's LDS: at most the device's per-workgroup
+    // limit (160 KB on gfx950); the default chunk keeps it within 64 KB, so
+    // that several workgroups share a CU (slit, N = 16384, 256^2 x 96: 8 rows
+    // in 62 KB 121 us, 16 rows in 71 KB 150 us); dps_batch_tune may pick
+    // longer chunks (the 1024^2 x 48 monolayer: 16 rows 2.5 ms, 8 rows 3.0)
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lim > 0)
+      h->lds_max = (size_t)lim;
     SpreadArgs S{};
     S.TX = h->TX; S.TY = h->TY; S.ZC = h->ZC; S.rs_max = std::max(h->rs_m, h->rs_d);
     if (h->has_dip && !h->reg)   // paired force-torque entries
       S.rs_max = std::max(S.rs_max, h->km.wx + h->km.wy + h->km.wzmax + 2 * (h->kd.wx + h->kd.wy + h->kd.wzmax) + 7);
     if (spread_npf(S.rs_max) == 0) return fail(DPS_ERR_ARG, "kernel z support too wide for the spreading records");
-    while (spread_lds_bytes(S, h->tsz) > 65536 && S.ZC > 1) S.ZC /= 2;
-    if (spread_lds_bytes(S, h->tsz) > 65536) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
+    while (spread_lds_bytes(S, h->tsz) > std::min<size_t>(65536, h->lds_max) && S.ZC > 1) S.ZC /= 2;
+    if (spread_lds_bytes(S, h->tsz) > h->lds_max) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
     h->ZC = S.ZC;
   }
   return DPS_OK;
@@ -840,7 +849,7 @@ bool spread_fits(dps_handle* h, bool torques, int zc)
   }
   for (int i = 0; i < n; ++i) {
     passes[i].ZC = zc;
-    if (spread_npf(passes[i].rs_max) == 0 || spread_lds_bytes(passes[i], h->tsz) > 65536) return false;
+    if (spread_npf(passes[i].rs_max) == 0 || spread_lds_bytes(passes[i], h->tsz) > h->lds_max) return false;
   }
   return true;
 }

@@ -389,11 +389,15 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   // lanes of one atomic instruction spread over the LDS banks
   constexpr int RS = TX + 1, PL = TY * RS + 1;
   // one dynamic LDS region, no static LDS: the accumulator starts at offset 0
-  // so that FP64 accumulators are 8-byte aligned for ds_add_f64
+  // so that it is 8-byte aligned for ds_add_f64.  The accumulator is FP64 in
+  // both precisions: on gfx950 an FP32 LDS atomic add (ds_add_f32) retires
+  // at ~193 cycles per 64-lane instruction per CU, an FP64 one (ds_add_f64)
+  // at ~9 (measured), so FP32 weights are summed in FP64 and rounded once on
+  // the way out
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  T* acc = reinterpret_cast<T*>(smem_raw);
+  double* acc = reinterpret_cast<double*>(smem_raw);
   const int vol = A.ZC * PL;
-  T* bst = acc + 3 * vol;                                  // [B][SLOT] staged entries
+  T* bst = reinterpret_cast<T*>(acc + 3 * vol);           // [B][SLOT] staged entries
   int* queue = reinterpret_cast<int*>(bst + B * SLOT);     // p * 2 + kind
   int* tstart = queue + 2 * 256;   // first candidate of each neighbour tile
   int* tpre = tstart + MAXT;       // prefix sums of the tiles' counts
@@ -402,25 +406,41 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
   const int X0 = tx * TX, Y0 = ty * TY, Z0 = zc * A.ZC;
   const int nX = min(TX, A.Nx - X0), nY = min(TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
-  for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (T)0;
+  for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = 0.0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
   const int nkind = Tq ? 2 : 1;
   const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
   const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
   const int ntl = nbx * nby;
-  if (threadIdx.x == 0) {
+  if (wave == 0) {
     // the neighbourhood's candidates as one list: tile j holds candidates
-    // [tpre[j], tpre[j+1]) starting at order position tstart[j]
-    int run = 0;
-    for (int j = 0; j < ntl; ++j) {
-      const int t2 = wrapi(by0 + j / nbx, A.nty) * A.ntx + wrapi(bx0 + j % nbx, A.ntx);
-      tstart[j] = A.start[t2];
-      tpre[j] = run;
-      run += A.start[t2 + 1] - A.start[t2];
+    // [tpre[j], tpre[j+1]) starting at order position tstart[j]; a lane per
+    // tile loads its range (all loads in flight together), a wave scan
+    // accumulates the counts
+    int carry = 0;
+    for (int j0 = 0; j0 < ntl; j0 += 64) {
+      const int j = j0 + lane;
+      int c = 0;
+      if (j < ntl) {
+        const int t2 = wrapi(by0 + j / nbx, A.nty) * A.ntx + wrapi(bx0 + j % nbx, A.ntx);
+        const int s0 = A.start[t2];
+        c = A.start[t2 + 1] - s0;
+        tstart[j] = s0;
+      }
+      int v = c;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+      }
+      if (j < ntl) tpre[j] = carry + v - c;
+      carry += __shfl(v, 63, 64);
     }
-    tpre[ntl] = run;
-    qn = 0;
+    if (lane == 0) {
+      tpre[ntl] = carry;
+      qn = 0;
+    }
   }
   __syncthreads();
   const int ncand = tpre[ntl];
@@ -574,7 +594,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               if (phase >= P || col >= L) continue;
               const int b = col / cx, a = col - b * cx;
               const int i = ix0 + a, j = iy0 + b;
-              T* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
+              double* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
               const int q1 = qb2;
               if (kind == 2) {
                 // force (monopole) and torque curl at each point, one atomic
@@ -598,10 +618,10 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                   }
                   const T w = wm * zm;
                   const T gx = ax * zA, gy = ay * zA, gz = az * zB;
-                  T* d = dcol + q * PL;
-                  atomicAdd(d, w * f0 + h * (gy * t2 - gz * t1));
-                  atomicAdd(d + vol, w * f1 + h * (gz * t0 - gx * t2));
-                  atomicAdd(d + 2 * vol, w * f2 + h * (gx * t1 - gy * t0));
+                  double* d = dcol + q * PL;
+                  atomicAdd(d, (double)(w * f0 + h * (gy * t2 - gz * t1)));
+                  atomicAdd(d + vol, (double)(w * f1 + h * (gz * t0 - gx * t2)));
+                  atomicAdd(d + 2 * vol, (double)(w * f2 + h * (gx * t1 - gy * t0)));
                 }
               } else if (kind == 0) {
                 const T wxy = wxr[i] * wyr[j];
@@ -610,19 +630,24 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                 for (; q + 3 * P < q1; q += 4 * P) {
                   const T z0 = zr[q], z1 = zr[q + P], z2 = zr[q + 2 * P], z3 = zr[q + 3 * P];
                   const T w0 = wxy * z0, w1 = wxy * z1, w2 = wxy * z2, w3 = wxy * z3;
-                  T* d0 = dcol + q * PL;
-                  atomicAdd(d0, w0 * g0); atomicAdd(d0 + vol, w0 * g1); atomicAdd(d0 + 2 * vol, w0 * g2);
-                  T* d1 = d0 + P * PL;
-                  atomicAdd(d1, w1 * g0); atomicAdd(d1 + vol, w1 * g1); atomicAdd(d1 + 2 * vol, w1 * g2);
-                  T* d2 = d1 + P * PL;
-                  atomicAdd(d2, w2 * g0); atomicAdd(d2 + vol, w2 * g1); atomicAdd(d2 + 2 * vol, w2 * g2);
-                  T* d3 = d2 + P * PL;
-                  atomicAdd(d3, w3 * g0); atomicAdd(d3 + vol, w3 * g1); atomicAdd(d3 + 2 * vol, w3 * g2);
+                  double* d0 = dcol + q * PL;
+                  atomicAdd(d0, (double)(w0 * g0)); atomicAdd(d0 + vol, (double)(w0 * g1));
+                  atomicAdd(d0 + 2 * vol, (double)(w0 * g2));
+                  double* d1 = d0 + P * PL;
+                  atomicAdd(d1, (double)(w1 * g0)); atomicAdd(d1 + vol, (double)(w1 * g1));
+                  atomicAdd(d1 + 2 * vol, (double)(w1 * g2));
+                  double* d2 = d1 + P * PL;
+                  atomicAdd(d2, (double)(w2 * g0)); atomicAdd(d2 + vol, (double)(w2 * g1));
+                  atomicAdd(d2 + 2 * vol, (double)(w2 * g2));
+                  double* d3 = d2 + P * PL;
+                  atomicAdd(d3, (double)(w3 * g0)); atomicAdd(d3 + vol, (double)(w3 * g1));
+                  atomicAdd(d3 + 2 * vol, (double)(w3 * g2));
                 }
                 for (; q < q1; q += P) {
                   const T w = wxy * zr[q];
-                  T* d = dcol + q * PL;
-                  atomicAdd(d, w * g0); atomicAdd(d + vol, w * g1); atomicAdd(d + 2 * vol, w * g2);
+                  double* d = dcol + q * PL;
+                  atomicAdd(d, (double)(w * g0)); atomicAdd(d + vol, (double)(w * g1));
+                  atomicAdd(d + 2 * vol, (double)(w * g2));
                 }
               } else {
                 const T h = (T)0.5;
@@ -632,26 +657,26 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                   const T zA0 = zr[q], zB0 = zr[W + q], zA1 = zr[q + P], zB1 = zr[W + q + P];
                   {
                     const T gx = ax * zA0, gy = ay * zA0, gz = az * zB0;
-                    T* d = dcol + q * PL;
-                    atomicAdd(d, h * (gy * f2 - gz * f1));
-                    atomicAdd(d + vol, h * (gz * f0 - gx * f2));
-                    atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                    double* d = dcol + q * PL;
+                    atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
+                    atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
+                    atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
                   }
                   {
                     const T gx = ax * zA1, gy = ay * zA1, gz = az * zB1;
-                    T* d = dcol + (q + P) * PL;
-                    atomicAdd(d, h * (gy * f2 - gz * f1));
-                    atomicAdd(d + vol, h * (gz * f0 - gx * f2));
-                    atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                    double* d = dcol + (q + P) * PL;
+                    atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
+                    atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
+                    atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
                   }
                 }
                 for (; q < q1; q += P) {
                   const T zA = zr[q], zB = zr[W + q];
                   const T gx = ax * zA, gy = ay * zA, gz = az * zB;
-                  T* d = dcol + q * PL;
-                  atomicAdd(d, h * (gy * f2 - gz * f1));
-                  atomicAdd(d + vol, h * (gz * f0 - gx * f2));
-                  atomicAdd(d + 2 * vol, h * (gx * f1 - gy * f0));
+                  double* d = dcol + q * PL;
+                  atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
+                  atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
+                  atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
                 }
               }
             }
@@ -672,10 +697,10 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
     if (ix < nX)
       for (int c = 0; c < 3; ++c) {
         T* gc = grid + ((size_t)c * A.Nz + Z0) * A.Ny * A.Nx + (size_t)Y0 * A.Nx + X0 + ix;
-        const T* ac = acc + c * vol + ix;
+        const double* ac = acc + c * vol + ix;
         for (int row = threadIdx.x / TX; row < nrow; row += rstep) {
           const int kz = row / TY, jy = row % TY;
-          if (jy < nY) gc[((size_t)kz * A.Ny + jy) * A.Nx] = ac[kz * PL + jy * RS];
+          if (jy < nY) gc[((size_t)kz * A.Ny + jy) * A.Nx] = (T)ac[kz * PL + jy * RS];
         }
       }
   }
@@ -1696,8 +1721,9 @@ int spread_npf(int rs_max)
 
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz)
 {
-  // accumulator, staged batch, queue, tile table + counter
-  return ((size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) + (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max)) * tsz +
+  // FP64 accumulator, staged batch, queue, tile table + counter
+  return (size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) * sizeof(double) +
+         (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max) * tsz +
          (2 * 256 + 2 * DPS_SPREAD_MAXT + 2) * sizeof(int);
 }
 
@@ -1708,10 +1734,20 @@ void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream
   dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
   const Tin *f = (const Tin*)F, *t = (const Tin*)Tq;
   switch (spread_npf(A.rs_max)) {
-#define SPREAD_CASE(NPF) \
-  case NPF: hipLaunchKernelGGL((k_spread<T, Tin, NPF>), grid, dim3(256), lds, s, A, f, t); break;
+#define SPREAD_CASE(NPF)                                                                             \
+  case NPF:                                                                                          \
+    if (lds > 65536)                                                                                 \
+      (void)hipFuncSetAttribute((const void*)k_spread<T, Tin, NPF>,                                  \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);               \
+    hipLaunchKernelGGL((k_spread<T, Tin, NPF>), grid, dim3(256), lds, s, A, f, t);                   \
+    break;
     SPREAD_CASE(8) SPREAD_CASE(10) SPREAD_CASE(12) SPREAD_CASE(16) SPREAD_CASE(20) SPREAD_CASE(24)
-    default: hipLaunchKernelGGL((k_spread<T, Tin, 32>), grid, dim3(256), lds, s, A, f, t); break;
+    default:
+      if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_spread<T, Tin, 32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+      hipLaunchKernelGGL((k_spread<T, Tin, 32>), grid, dim3(256), lds, s, A, f, t);
+      break;
 #undef SPREAD_CASE
   }
 }
