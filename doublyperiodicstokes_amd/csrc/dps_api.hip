@@ -104,7 +104,8 @@ struct dps_handle {
   std::vector<double> zpts, zwts;
   ChebTables cheb;
   ChebTransform ct;
-  int TX = 16, TY = 4, ZC = 32, ntx = 0, nty = 0, ntiles = 0, RX = 1, RY = 1;
+  int TX = 16, TY = 4, ZC = 32, ntx = 0, nty = 0, ntiles = 0;
+  int zc_min = 8;   // shortest z chunk the spreading lists are sized for
   size_t lds_max = 65536;   // LDS bytes one workgroup may use on this device
   size_t tsz = 8;   // bytes per grid real
 
@@ -116,6 +117,16 @@ struct dps_handle {
   void *d_rec_m = nullptr, *d_rec_d = nullptr;
   int rs_m = 0, rs_d = 0;
   int4* d_zrange = nullptr;
+  // spreading entry lists of each pass (rebuilt whenever positions change):
+  // 0 forces, 1 forces and torques with the derivative kernels, 2 torques
+  // with the regular kernel
+  struct SpreadLists {
+    bool used = false;
+    int nkind = 1, rows0 = 0;
+    bool dip0 = false;   // kind-0 entries use the torque kernel's supports
+    int *start = nullptr, *counts = nullptr, *fill = nullptr, *entries = nullptr;
+    int cap = 0;
+  } lists[3];
   int *d_key = nullptr, *d_rank = nullptr, *d_counts = nullptr, *d_start = nullptr, *d_order = nullptr;
   void* d_grid = nullptr;
   void *d_specA = nullptr, *d_specB = nullptr;
@@ -372,10 +383,6 @@ int build_tiles(dps_handle* h, int wmax)
   h->ntx = (h->Nx + h->TX - 1) / h->TX;
   h->nty = (h->Ny + h->TY - 1) / h->TY;
   h->ntiles = h->ntx * h->nty;
-  h->RX = (wmax + 1 + h->TX - 1) / h->TX + ((h->Nx % h->TX) ? 1 : 0);
-  h->RY = (wmax + 1 + h->TY - 1) / h->TY + ((h->Ny % h->TY) ? 1 : 0);
-  if (std::min(2 * h->RX + 1, h->ntx) * std::min(2 * h->RY + 1, h->nty) > DPS_SPREAD_MAXT)
-    return fail(DPS_ERR_ARG, "kernel support too wide for the spreading tiles");
   // particle weight records: mono [wx][wy][z spread][z interp]; dipole
   // [wx][wy][dx][dy][z spread][dz spread][z interp][dz interp], z parts by
   // interior row from sup.z
@@ -384,11 +391,9 @@ int build_tiles(dps_handle* h, int wmax)
             : h->reg ? h->kd.wx + h->kd.wy + 2 * h->kd.wzmax
                      : 2 * h->kd.wx + 2 * h->kd.wy + 4 * h->kd.wzmax;
   {
-    // the spreading workgroup's LDS: at most the device's per-workgroup
-    // limit (160 KB on gfx950); the default chunk keeps it within 64 KB, so
+    // the spreading workgroup's LDS stays within 64 KB (spread_lds_cap), so
     // that several workgroups share a CU (slit, N = 16384, 256^2 x 96: 8 rows
-    // in 62 KB 121 us, 16 rows in 71 KB 150 us); dps_batch_tune may pick
-    // longer chunks (the 1024^2 x 48 monolayer: 16 rows 2.5 ms, 8 rows 3.0)
+    // in 62 KB 121 us, 16 rows in 71 KB 150 us)
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lim > 0)
@@ -399,8 +404,30 @@ int build_tiles(dps_handle* h, int wmax)
       S.rs_max = std::max(S.rs_max, h->km.wx + h->km.wy + h->km.wzmax + 2 * (h->kd.wx + h->kd.wy + h->kd.wzmax) + 7);
     if (spread_npf(S.rs_max) == 0) return fail(DPS_ERR_ARG, "kernel z support too wide for the spreading records");
     while (spread_lds_bytes(S, h->tsz) > std::min<size_t>(65536, h->lds_max) && S.ZC > 1) S.ZC /= 2;
-    if (spread_lds_bytes(S, h->tsz) > h->lds_max) return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
+    if (spread_lds_bytes(S, h->tsz) > std::min<size_t>(65536, h->lds_max))
+      return fail(DPS_ERR_ARG, "spreading tile does not fit in LDS");
     h->ZC = S.ZC;
+  }
+  // the spreading passes and their entry lists
+  h->zc_min = std::min(h->ZC, 8);
+  h->lists[0].used = true;
+  if (h->has_dip && !h->reg) {
+    h->lists[1].used = true;
+    const bool pair = h->km.wx == h->kd.wx && h->km.wy == h->kd.wy;
+    h->lists[1].nkind = pair ? 1 : 2;
+    h->lists[1].rows0 = pair ? 2 : 0;
+  }
+  if (h->has_dip && h->reg) {
+    h->lists[2].used = true;
+    h->lists[2].rows0 = 1;
+    h->lists[2].dip0 = true;
+  }
+  for (const auto& L : h->lists) {
+    if (!L.used) continue;
+    const int S0 = L.rows0 == 0 ? h->km.wzmax : L.rows0 == 1 ? h->kd.wzmax : h->km.wzmax + h->kd.wzmax;
+    const int S = std::max(S0, L.nkind > 1 ? h->kd.wzmax : 0);
+    if ((S + h->zc_min - 1) / h->zc_min + 2 > DPS_LIST_MAXC)
+      return fail(DPS_ERR_ARG, "kernel z support too long for the spreading lists");
   }
   return DPS_OK;
 }
@@ -437,6 +464,24 @@ int allocate(dps_handle* h)
   if ((rc = h->alloc(&h->d_order, N))) return rc;
   if ((rc = h->alloc(&h->d_counts, h->ntiles))) return rc;
   if ((rc = h->alloc(&h->d_start, h->ntiles + 1))) return rc;
+  {
+    // spreading lists, sized for the shortest z chunk
+    const size_t ncells = (size_t)h->ntiles * ((h->Nz + h->zc_min - 1) / h->zc_min);
+    for (auto& L : h->lists) {
+      if (!L.used) continue;
+      const KernelDesc& K0 = L.dip0 ? h->kd : h->km;
+      const int S0 = L.rows0 == 0 ? h->km.wzmax : L.rows0 == 1 ? h->kd.wzmax : h->km.wzmax + h->kd.wzmax;
+      size_t per = list_cells_bound(K0.wx, K0.wy, S0, h->TX, h->TY, h->zc_min);
+      if (L.nkind > 1) per += list_cells_bound(h->kd.wx, h->kd.wy, h->kd.wzmax, h->TX, h->TY, h->zc_min);
+      const size_t cap = std::max<size_t>(1, (size_t)N * per);
+      if (cap > (size_t)INT32_MAX) return fail(DPS_ERR_ARG, "too many spreading entries");
+      L.cap = (int)cap;
+      if ((rc = h->alloc(&L.start, ncells + 1))) return rc;
+      if ((rc = h->alloc(&L.counts, ncells))) return rc;
+      if ((rc = h->alloc(&L.fill, ncells))) return rc;
+      if ((rc = h->alloc(&L.entries, cap))) return rc;
+    }
+  }
   if ((rc = h->alloc((unsigned char**)&h->d_grid, ng * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specA, nspec * 2 * ts))) return rc;
   if ((rc = h->alloc((unsigned char**)&h->d_specB, nspec * 2 * ts))) return rc;
@@ -697,8 +742,11 @@ SpreadArgs spread_args(dps_handle* h)
 {
   SpreadArgs A{};
   A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
-  A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty; A.RX = h->RX; A.RY = h->RY;
-  A.start = h->d_start; A.order = h->d_order;
+  A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty;
+  A.nch = (h->Nz + h->ZC - 1) / h->ZC;
+  A.lstart = h->lists[0].start;
+  A.lentries = h->lists[0].entries;
+  A.lcap = h->lists[0].cap;
   A.zrange = h->d_zrange;
   A.km = h->km; A.kd = h->kd;
   A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
@@ -719,9 +767,19 @@ bool spread_pairable(const dps_handle* h)
   return h->has_dip && !h->reg && h->km.wx == h->kd.wx && h->km.wy == h->kd.wy;
 }
 
-SpreadArgs spread_args_pair(dps_handle* h)
+// the pass with torques through the derivative kernels (paired or not)
+SpreadArgs spread_args_torques(dps_handle* h)
 {
   SpreadArgs A = spread_args(h);
+  A.lstart = h->lists[1].start;
+  A.lentries = h->lists[1].entries;
+  A.lcap = h->lists[1].cap;
+  return A;
+}
+
+SpreadArgs spread_args_pair(dps_handle* h)
+{
+  SpreadArgs A = spread_args_torques(h);
   A.pair = 1;
   A.pm = h->km.wx + h->km.wy + h->km.wzmax;
   A.pd = 2 * h->kd.wx + 2 * h->kd.wy + 2 * h->kd.wzmax;
@@ -741,7 +799,37 @@ SpreadArgs spread_args_torque(dps_handle* h)
   A.rs_m = h->rs_d;
   A.grid = h->d_gridT;
   A.zr_dip = 1;
+  A.lstart = h->lists[2].start;
+  A.lentries = h->lists[2].entries;
+  A.lcap = h->lists[2].cap;
   return A;
+}
+
+// the spreading lists of every pass, for the located positions and the
+// current z chunk (kernels only: a captured position update rebuilds them)
+int build_lists(dps_handle* h, hipStream_t s)
+{
+  for (int i = 0; i < 3; ++i) {
+    auto& L = h->lists[i];
+    if (!L.used) continue;
+    ListArgs A{};
+    A.N = h->N; A.Nx = h->Nx; A.Ny = h->Ny; A.Nz = h->Nz;
+    A.TX = h->TX; A.TY = h->TY; A.ZC = h->ZC; A.ntx = h->ntx; A.nty = h->nty;
+    A.nch = (h->Nz + h->ZC - 1) / h->ZC;
+    A.periodic_z = h->geometry == DPS_MODE_PERIODIC;
+    A.nkind = L.nkind;
+    A.rows0 = L.rows0;
+    const KernelDesc& K0 = L.dip0 ? h->kd : h->km;
+    A.wx0 = K0.wx; A.wy0 = K0.wy; A.wx1 = h->kd.wx; A.wy1 = h->kd.wy;
+    A.sup0 = L.dip0 ? h->d_sup_d : h->d_sup_m;
+    A.sup1 = h->d_sup_d;
+    A.zrange = h->d_zrange;
+    A.counts = L.counts; A.fill = L.fill; A.start = L.start; A.entries = L.entries;
+    A.cap = L.cap;
+    launch_lists(A, s);
+  }
+  HIPCHK(hipGetLastError());
+  return DPS_OK;
 }
 
 InterpArgs interp_args(dps_handle* h)
@@ -833,6 +921,12 @@ void spread_pass(const SpreadArgs& A, const void* F, const void* Tq, bool mx, in
 // forces (and the torque curl of the derivative kernels) into d_grid; with
 // the regular torque kernel the torques go, as a plain vector field, into
 // d_gridT
+// LDS budget of a spreading workgroup: 64 KB.  The device allows more per
+// workgroup (160 KB on gfx950) and eager launches take it, but a batch
+// graph replaying such a launch produced wrong grids, so no spreading
+// configuration goes beyond 64 KB.
+size_t spread_lds_cap(const dps_handle* h) { return std::min<size_t>(65536, h->lds_max); }
+
 // whether every spreading pass do_spread runs (with or without torques) fits
 // the LDS budget with z chunks of zc levels
 bool spread_fits(dps_handle* h, bool torques, int zc)
@@ -849,7 +943,7 @@ bool spread_fits(dps_handle* h, bool torques, int zc)
   }
   for (int i = 0; i < n; ++i) {
     passes[i].ZC = zc;
-    if (spread_npf(passes[i].rs_max) == 0 || spread_lds_bytes(passes[i], h->tsz) > h->lds_max) return false;
+    if (spread_npf(passes[i].rs_max) == 0 || spread_lds_bytes(passes[i], h->tsz) > spread_lds_cap(h)) return false;
   }
   return true;
 }
@@ -863,7 +957,7 @@ int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_
   } else if (Tq && spread_pairable(h)) {
     spread_pass(spread_args_pair(h), F, Tq, mixed(h), dtype, s);
   } else {
-    spread_pass(spread_args(h), F, Tq, mixed(h), dtype, s);
+    spread_pass(Tq ? spread_args_torques(h) : spread_args(h), F, Tq, mixed(h), dtype, s);
   }
   HIPCHK(hipGetLastError());
   return DPS_OK;
@@ -1314,6 +1408,8 @@ int dps_set_positions(dps_handle* h, const void* positions, int dtype, void* str
   HIPCHK(hipGetLastError());
   launch_sort(h->d_key, h->d_counts, h->d_rank, h->d_start, h->d_order, h->N, h->ntiles, s);
   HIPCHK(hipGetLastError());
+  int rcl = build_lists(h, s);
+  if (rcl) return rcl;
   HIPCHK(hipMemcpyAsync(h->h_err, h->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (*h->h_err) {
@@ -1354,7 +1450,7 @@ int dps_update_positions(dps_handle* h, void* positions, const void* velocities,
   HIPCHK(hipGetLastError());
   launch_sort(h->d_key, h->d_counts, h->d_rank, h->d_start, h->d_order, h->N, h->ntiles, s);
   HIPCHK(hipGetLastError());
-  return DPS_OK;
+  return build_lists(h, s);
 }
 
 int dps_check_positions(dps_handle* h, void* stream)
@@ -1630,7 +1726,7 @@ int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* t
     (void)hipEventDestroy(e0);
     return fail(DPS_ERR_HIP, "event creation failed");
   }
-  static const int cands[] = {4, 8, 12, 16, 24, 32, 48};
+  static const int cands[] = {8, 12, 16, 24, 32, 48};
   int rc = DPS_OK;
   for (size_t i = 0; i < b->hs.size() && !rc; ++i) {
     dps_handle* h = b->hs[i];
@@ -1643,8 +1739,9 @@ int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* t
     float best_ms = 1e30f;
     for (int zc : cands) {
       // the candidate must fit the passes do_spread will run for these inputs
-      if (zc > h->Nz || !spread_fits(h, t != nullptr, zc)) continue;
+      if (zc > h->Nz || zc < h->zc_min || !spread_fits(h, t != nullptr, zc)) continue;
       h->ZC = zc;
+      if ((rc = build_lists(h, s))) break;
       // one untimed call, then the spreading of three calls timed together
       if ((rc = do_spread(h, forces[i], t, dtype, s))) break;
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(DPS_ERR_HIP, "event record failed"); break; }
@@ -1659,6 +1756,7 @@ int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* t
       if (ms < best_ms) { best_ms = ms; best = zc; }
     }
     h->ZC = best;
+    if (!rc) rc = build_lists(h, s);
     if (chunk_out) chunk_out[i] = best;
   }
   (void)hipEventDestroy(e0);

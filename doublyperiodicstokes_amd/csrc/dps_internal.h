@@ -5,7 +5,7 @@
 #include "dps_core.h"
 
 #define DPS_MAX_W 32          // largest x/y support (points) a stencil may have
-#define DPS_SPREAD_MAXT 128   // largest number of neighbour tiles a spreading workgroup scans
+#define DPS_LIST_MAXC 64       // z chunks one spreading entry may reach
 #define DPS_SPREAD_BATCH 16   // queued spreading entries staged together
 // spreading tile: SP_TX x SP_TY x-y cells
 #define SP_TX 16
@@ -60,8 +60,12 @@ struct StencilArgs {
 struct SpreadArgs {
   int Nx, Ny, Nz;
   int periodic_z;   // rows of a support wrap modulo Nz
-  int TX, TY, ZC, ntx, nty, RX, RY;
-  const int *start, *order;
+  int TX, TY, ZC, ntx, nty;
+  int nch;          // z chunks of ZC rows
+  // entries (2 p + kind) of every (tile, chunk) cell, cell = (ty ntx + tx) nch + chunk:
+  // lentries[lstart[cell] .. lstart[cell + 1])
+  const int *lstart, *lentries;
+  int lcap;         // capacity of lentries
   const int4* zrange;
   KernelDesc km, kd;
   const int4 *sup_m, *sup_d;
@@ -78,6 +82,27 @@ struct SpreadArgs {
   int ho;       // header offset in a staged entry
 };
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
+
+// Spreading entry lists of one pass, built from the located particles: every
+// (particle, kind) entry is listed in each (tile, z chunk) cell its support
+// reaches.  rows0: z rows of kind-0 entries, 0 = zrange.x..y (force kernel),
+// 1 = zrange.z..w (torque kernel), 2 = the union of both (paired entries);
+// kind-1 entries (nkind = 2) use zrange.z..w and sup1.
+struct ListArgs {
+  int N, Nx, Ny, Nz, TX, TY, ZC, ntx, nty, nch;
+  int periodic_z;
+  int nkind, rows0;
+  int wx0, wy0, wx1, wy1;
+  const int4 *sup0, *sup1, *zrange;
+  int *counts, *fill, *start, *entries;
+  int cap;   // capacity of entries
+};
+void launch_lists(const ListArgs& A, hipStream_t s);
+// upper bound of the cells one entry of support w (points) and z span S (rows) reaches
+inline int list_cells_bound(int wx, int wy, int S, int TX, int TY, int ZC)
+{
+  return ((wx + TX - 1) / TX + 2) * ((wy + TY - 1) / TY + 2) * ((S + ZC - 1) / ZC + 2);
+}
 // words staged per thread and spreading entry for records of rs_max words (0: too long)
 int spread_npf(int rs_max);
 

@@ -336,6 +336,71 @@ __global__ void k_tile_scatter(const int* __restrict__ key, const int* __restric
 }
 
 // ---------------------------------------------------------------------------
+// spreading entry lists: (tile, z chunk) cells of every entry
+// ---------------------------------------------------------------------------
+constexpr int LIST_MAXT = 8;   // distinct tiles along x or y (support <= DPS_MAX_W points)
+constexpr int LIST_MAXC = DPS_LIST_MAXC;   // distinct chunks along z
+
+// distinct tiles of size T met by the periodic run b .. b + w - 1 of n points
+// (a run never covers the period twice: n >= 2 w)
+__device__ __forceinline__ int run_tiles(int b, int w, int n, int T, int* out, int maxn)
+{
+  int k = 0;
+  for (int i = 0; i < w; ++i) {
+    const int t = wrapi(b + i, n) / T;
+    bool seen = false;
+    for (int j = 0; j < k; ++j) seen = seen || out[j] == t;
+    if (!seen && k < maxn) out[k++] = t;
+  }
+  return k;
+}
+
+// the cells of entry (p, kind): x tiles, y tiles, z chunks
+__device__ __forceinline__ void entry_cells(const ListArgs& A, int p, int kind, int* tx, int* nx, int* ty,
+                                            int* ny, int* ch, int* nc)
+{
+  const int4 sup = (kind ? A.sup1 : A.sup0)[p];
+  const int4 zr = A.zrange[p];
+  int klo, khi;
+  if (kind || A.rows0 == 1) {
+    klo = zr.z;
+    khi = zr.w;
+  } else {
+    klo = zr.x;
+    khi = zr.y;
+    if (A.rows0 == 2 && zr.z <= zr.w) {
+      klo = min(klo, zr.z);
+      khi = max(khi, zr.w);
+    }
+  }
+  *nx = run_tiles(sup.x, kind ? A.wx1 : A.wx0, A.Nx, A.TX, tx, LIST_MAXT);
+  *ny = run_tiles(sup.y, kind ? A.wy1 : A.wy0, A.Ny, A.TY, ty, LIST_MAXT);
+  *nc = (khi >= klo) ? run_tiles(klo, khi - klo + 1, A.periodic_z ? A.Nz : (1 << 30), A.ZC, ch, LIST_MAXC) : 0;
+}
+
+template <bool SCATTER>
+__global__ void k_list_cells(ListArgs A)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.N * A.nkind) return;
+  const int p = i / A.nkind, kind = i - p * A.nkind;
+  int tx[LIST_MAXT], ty[LIST_MAXT], ch[LIST_MAXC], nx, ny, nc;
+  entry_cells(A, p, kind, tx, &nx, ty, &ny, ch, &nc);
+  for (int a = 0; a < ny; ++a)
+    for (int b = 0; b < nx; ++b)
+      for (int c = 0; c < nc; ++c) {
+        if (ty[a] >= A.nty || tx[b] >= A.ntx || ch[c] < 0 || ch[c] >= A.nch) continue;   // cannot happen
+        const int cell = (ty[a] * A.ntx + tx[b]) * A.nch + ch[c];
+        if (!SCATTER) {
+          atomicAdd(&A.counts[cell], 1);
+        } else {
+          const int at = A.start[cell] + atomicAdd(&A.fill[cell], 1);
+          if (at < A.cap) A.entries[at] = 2 * p + kind;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // spread
 // ---------------------------------------------------------------------------
 // one contiguous piece of a periodic support range intersected with a tile:
@@ -364,10 +429,10 @@ __device__ __forceinline__ int fdiv(int t, int n, float inv, int* rem)
   return c;
 }
 
-// Spreading.  One workgroup owns a tile of SP_TX x SP_TY columns and ZC
-// interior z rows.  The particles of the tile's neighbourhood are culled in
-// parallel (one thread per candidate) into an LDS queue of (particle, kernel)
-// entries whose support reaches the tile.  The entries are staged in batches,
+// Spreading.  One workgroup owns a cell: a tile of SP_TX x SP_TY columns and
+// ZC interior z rows.  The cell's (particle, kernel) entries, those whose
+// support reaches it, were listed when the positions were set (k_list_cells);
+// they are read into an LDS queue.  The entries are staged in batches,
 // 16 threads per entry loading its weight record, support and force, one
 // batch ahead of the batch being spread, so a batch's global loads are in
 // flight together.  A wavefront spreads one entry at a time: each lane owns
@@ -381,7 +446,6 @@ template <class T, class Tin, int NPF>
 __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restrict__ F,
                                                 const Tin* __restrict__ Tq)
 {
-  constexpr int MAXT = DPS_SPREAD_MAXT;   // neighbour tiles a workgroup scans (checked on the host)
   constexpr int B = DPS_SPREAD_BATCH;     // queued entries staged per batch, 16 threads each
   constexpr int SLOT = 16 * NPF;          // words per staged entry: record, then header at rs_max
   constexpr int TX = SP_TX, TY = SP_TY;
@@ -399,51 +463,23 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   const int vol = A.ZC * PL;
   T* bst = reinterpret_cast<T*>(acc + 3 * vol);           // [B][SLOT] staged entries
   int* queue = reinterpret_cast<int*>(bst + B * SLOT);     // p * 2 + kind
-  int* tstart = queue + 2 * 256;   // first candidate of each neighbour tile
-  int* tpre = tstart + MAXT;       // prefix sums of the tiles' counts
-  int& qn = tpre[MAXT + 1];
   T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
-  const int tx = blockIdx.x, ty = blockIdx.y, zc = blockIdx.z;
+  // cells in list order, consecutive cells (the chunks of a tile, then the
+  // next tile) on one XCD, whose L2 then holds their shared records
+  const int cell = xcd_group(blockIdx.x, gridDim.x);
+  const int zc = cell % A.nch, tile = cell / A.nch;
+  const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int X0 = tx * TX, Y0 = ty * TY, Z0 = zc * A.ZC;
   const int nX = min(TX, A.Nx - X0), nY = min(TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
   for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = 0.0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
-  const int nkind = Tq ? 2 : 1;
-  const int nbx = min(2 * A.RX + 1, A.ntx), nby = min(2 * A.RY + 1, A.nty);
-  const int bx0 = (nbx == A.ntx) ? 0 : tx - A.RX, by0 = (nby == A.nty) ? 0 : ty - A.RY;
-  const int ntl = nbx * nby;
-  if (wave == 0) {
-    // the neighbourhood's candidates as one list: tile j holds candidates
-    // [tpre[j], tpre[j+1]) starting at order position tstart[j]; a lane per
-    // tile loads its range (all loads in flight together), a wave scan
-    // accumulates the counts
-    int carry = 0;
-    for (int j0 = 0; j0 < ntl; j0 += 64) {
-      const int j = j0 + lane;
-      int c = 0;
-      if (j < ntl) {
-        const int t2 = wrapi(by0 + j / nbx, A.nty) * A.ntx + wrapi(bx0 + j % nbx, A.ntx);
-        const int s0 = A.start[t2];
-        c = A.start[t2 + 1] - s0;
-        tstart[j] = s0;
-      }
-      int v = c;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-      }
-      if (j < ntl) tpre[j] = carry + v - c;
-      carry += __shfl(v, 63, 64);
-    }
-    if (lane == 0) {
-      tpre[ntl] = carry;
-      qn = 0;
-    }
-  }
+  // this cell's entries, listed when the positions were set
+  const int e0 = A.lstart[cell];
+  const int ncand = max(0, min(A.lstart[cell + 1], A.lcap) - e0);   // within the list's capacity
+  // the zeroed accumulator is complete before any thread adds to it or, for
+  // a cell without entries, writes it out
   __syncthreads();
-  const int ncand = tpre[ntl];
   // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2; paired
   // entries: force support, torque support, force, torque
   const int HO = A.ho;
@@ -486,40 +522,12 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
       v[k] = xv;
     }
   };
-  for (int c0 = 0; c0 < ncand; c0 += blockDim.x) {
-    // ---- cull: one candidate particle per thread ----
-    const int ci = c0 + threadIdx.x;
-    if (ci < ncand) {
-      int j = 0;
-      while (j + 1 < ntl && tpre[j + 1] <= ci) ++j;
-      const int p = A.order[tstart[j] + (ci - tpre[j])];
-      const int4 zr = A.zrange[p];
-      for (int kind = 0; kind < (A.pair ? 1 : nkind); ++kind) {
-        const bool zd = kind || A.zr_dip;
-        int klo = zd ? zr.z : zr.x, khi = zd ? zr.w : zr.y;
-        if (A.pair && zr.z <= zr.w) {   // the union of both kernels' rows
-          klo = min(klo, zr.z);
-          khi = max(khi, zr.w);
-        }
-        // rows klo..khi, and their periodic images when z is periodic
-        bool hz = false;
-        for (int sh = A.periodic_z ? -1 : 0; sh <= (A.periodic_z ? 1 : 0); ++sh)
-          hz = hz || !(khi + sh * A.Nz < Z0 || klo + sh * A.Nz >= Z0 + nZ);
-        if (!hz) continue;
-        const int4 sup = (kind ? A.sup_d : A.sup_m)[p];
-        const int wx = kind ? A.kd.wx : A.km.wx, wy = kind ? A.kd.wy : A.km.wy;
-        bool hx = false, hy = false;
-        int i0, l0;
-        for (int sh = -1; sh <= 1; ++sh) {
-          hx = hx || clip_piece(sup.x, wx, X0, nX, A.Nx, sh * A.Nx, &i0, &l0) > 0;
-          hy = hy || clip_piece(sup.y, wy, Y0, nY, A.Ny, sh * A.Ny, &i0, &l0) > 0;
-        }
-        if (hx && hy) queue[atomicAdd(&qn, 1)] = 2 * p + kind;
-      }
-    }
+  constexpr int QN = 2 * 256;   // queue capacity
+  for (int c0 = 0; c0 < ncand; c0 += QN) {
+    const int nq_entries = min(QN, ncand - c0);
+    for (int i = threadIdx.x; i < nq_entries; i += blockDim.x) queue[i] = A.lentries[e0 + c0 + i];
     __syncthreads();
     // ---- spread the queued entries in staged batches, one wavefront per entry ----
-    const int nq_entries = qn;
     T pf[NPF];
     prefetch(0, min(B, nq_entries), pf);
     for (int qb = 0; qb < nq_entries; qb += B) {
@@ -684,10 +692,8 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
         }
         }
       }
-      __syncthreads();   // the batch's slots are free again
+      __syncthreads();   // the batch's slots and the queue are free again
     }
-    if (threadIdx.x == 0) qn = 0;
-    __syncthreads();
   }
   // one write per grid element of the tile: thread -> fixed x, rows (z, y)
   // in steps of blockDim / TX
@@ -1710,6 +1716,17 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
                        order, N);
 }
 
+void launch_lists(const ListArgs& A, hipStream_t s)
+{
+  const int ncells = A.ntx * A.nty * A.nch;
+  const int ne = A.N * A.nkind;
+  hipLaunchKernelGGL(k_fill_int, dim3((ncells + 255) / 256), dim3(256), 0, s, A.counts, ncells, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3((ncells + 255) / 256), dim3(256), 0, s, A.fill, ncells, 0);
+  if (ne > 0) hipLaunchKernelGGL(k_list_cells<false>, dim3((ne + 255) / 256), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, A.counts, A.start, ncells);
+  if (ne > 0) hipLaunchKernelGGL(k_list_cells<true>, dim3((ne + 255) / 256), dim3(256), 0, s, A);
+}
+
 int spread_npf(int rs_max)
 {
   // staged words per entry: record + 7 header words, 16 threads per entry
@@ -1721,17 +1738,16 @@ int spread_npf(int rs_max)
 
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz)
 {
-  // FP64 accumulator, staged batch, queue, tile table + counter
+  // FP64 accumulator, staged batch, queue
   return (size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) * sizeof(double) +
-         (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max) * tsz +
-         (2 * 256 + 2 * DPS_SPREAD_MAXT + 2) * sizeof(int);
+         (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max) * tsz + 2 * 256 * sizeof(int);
 }
 
 template <class T, class Tin>
 void launch_spread(const SpreadArgs& A, const void* F, const void* Tq, hipStream_t s)
 {
   const size_t lds = spread_lds_bytes(A, sizeof(T));
-  dim3 grid(A.ntx, A.nty, (A.Nz + A.ZC - 1) / A.ZC);
+  dim3 grid(A.ntx * A.nty * A.nch);
   const Tin *f = (const Tin*)F, *t = (const Tin*)Tq;
   switch (spread_npf(A.rs_max)) {
 #define SPREAD_CASE(NPF)                                                                             \
