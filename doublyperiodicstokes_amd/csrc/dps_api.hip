@@ -117,6 +117,7 @@ struct dps_handle {
   void *d_rec_m = nullptr, *d_rec_d = nullptr;
   int rs_m = 0, rs_d = 0;
   int4* d_zrange = nullptr;
+  float* d_amax = nullptr;   // partial maxima of |F|, |T| for fixed-point spreading
   // spreading entry lists of each pass (rebuilt whenever positions change):
   // 0 forces, 1 forces and torques with the derivative kernels, 2 torques
   // with the regular kernel
@@ -260,6 +261,17 @@ int setup_kernel(dps_handle* h, int w, double alpha, double beta, bool dipole, K
   K->inv_norm3 = 1.0 / (norm * (norm * norm));
   K->thresh = dipole ? dipole_thresh(a, betaw) : a;
   if (!(K->thresh > 0) || !std::isfinite(K->thresh)) return fail(DPS_ERR_ARG, "invalid kernel cutoff");
+  // |phi'| = phi |dfac| is smooth on [0, thresh] for the dipole cutoff
+  // (thresh < alpha): a sampled maximum with a margin bounds it
+  K->dmax = 0.0;
+  if (dipole) {
+    const double a2 = a * a;
+    for (int i = 0; i <= 4096; ++i) {
+      const double x = K->thresh * i / 4096.0;
+      K->dmax = std::max(K->dmax, std::fabs(es_phi(x, betaw, a2) * es_dfac(x, betaw, a2)));
+    }
+    K->dmax *= 1.05;
+  }
   K->wx = (int)std::lround(2 * K->thresh / h->hx);
   K->wy = (int)std::lround(2 * K->thresh / h->hy);
   if (K->wx < 1 || K->wy < 1 || K->wx > DPS_MAX_W || K->wy > DPS_MAX_W)
@@ -459,6 +471,7 @@ int allocate(dps_handle* h)
     if ((rc = h->alloc((unsigned char**)&h->d_rec_d, (size_t)N * h->rs_d * ts))) return rc;
   }
   if ((rc = h->alloc(&h->d_zrange, N))) return rc;
+  if ((rc = h->alloc(&h->d_amax, 2 * DPS_AMAX_BLOCKS))) return rc;
   if ((rc = h->alloc(&h->d_key, N))) return rc;
   if ((rc = h->alloc(&h->d_rank, N))) return rc;
   if ((rc = h->alloc(&h->d_order, N))) return rc;
@@ -747,6 +760,10 @@ SpreadArgs spread_args(dps_handle* h)
   A.lstart = h->lists[0].start;
   A.lentries = h->lists[0].entries;
   A.lcap = h->lists[0].cap;
+  A.amax = h->d_amax;
+  A.namax = DPS_AMAX_BLOCKS;
+  A.K0 = 3 * h->km.inv_norm3;
+  A.K1 = h->has_dip ? 3 * h->kd.dmax * h->kd.inv_norm3 : 0.0;
   A.zrange = h->d_zrange;
   A.km = h->km; A.kd = h->kd;
   A.sup_m = h->d_sup_m; A.sup_d = h->d_sup_d;
@@ -799,6 +816,8 @@ SpreadArgs spread_args_torque(dps_handle* h)
   A.rs_m = h->rs_d;
   A.grid = h->d_gridT;
   A.zr_dip = 1;
+  A.K0 = 3 * h->kd.inv_norm3;   // the torques are this pass's forces
+  A.K1 = 0.0;
   A.lstart = h->lists[2].start;
   A.lentries = h->lists[2].entries;
   A.lcap = h->lists[2].cap;
@@ -907,9 +926,12 @@ void mark(dps_handle* h, int i, hipStream_t s)
   if (h->prof) (void)hipEventRecord(h->ev[i], s);
 }
 
-void spread_pass(const SpreadArgs& A, const void* F, const void* Tq, bool mx, int dtype, hipStream_t s)
+void spread_pass(const SpreadArgs& A, const void* F, const void* Tq, bool mx, int dtype, hipStream_t s, int n)
 {
   if (mx) {
+    // fixed-point accumulation: the pass's input maxima first
+    if (dtype == DPS_F32) launch_absmax<float>(F, Tq, 3 * n, const_cast<float*>(A.amax), s);
+    else launch_absmax<double>(F, Tq, 3 * n, const_cast<float*>(A.amax), s);
     if (dtype == DPS_F32) launch_spread<float, float>(A, F, Tq, s);
     else launch_spread<float, double>(A, F, Tq, s);
   } else {
@@ -952,12 +974,12 @@ int do_spread(dps_handle* h, const void* F, const void* T, int dtype, hipStream_
 {
   const void* Tq = (T && h->has_dip) ? T : nullptr;
   if (h->reg) {
-    spread_pass(spread_args(h), F, nullptr, mixed(h), dtype, s);
-    if (Tq) spread_pass(spread_args_torque(h), Tq, nullptr, mixed(h), dtype, s);
+    spread_pass(spread_args(h), F, nullptr, mixed(h), dtype, s, h->N);
+    if (Tq) spread_pass(spread_args_torque(h), Tq, nullptr, mixed(h), dtype, s, h->N);
   } else if (Tq && spread_pairable(h)) {
-    spread_pass(spread_args_pair(h), F, Tq, mixed(h), dtype, s);
+    spread_pass(spread_args_pair(h), F, Tq, mixed(h), dtype, s, h->N);
   } else {
-    spread_pass(Tq ? spread_args_torques(h) : spread_args(h), F, Tq, mixed(h), dtype, s);
+    spread_pass(Tq ? spread_args_torques(h) : spread_args(h), F, Tq, mixed(h), dtype, s, h->N);
   }
   HIPCHK(hipGetLastError());
   return DPS_OK;

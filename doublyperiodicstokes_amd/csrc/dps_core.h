@@ -70,6 +70,7 @@ struct KernelDesc {
   int nzeff;         // points of the z-extended grid
   int ext_up;        // mirrored ghost nodes above the top
   int wzmax;         // bound on the z support size
+  double dmax;       // bound on |phi'| over |x| <= thresh (dipole kernels; 0 otherwise)
 };
 
 // first index e with zext[e] <= v for descending zext (std::lower_bound with

@@ -80,8 +80,17 @@ struct SpreadArgs {
   // header at ho: both supports, the force, the torque
   int pair, pm, pd;
   int ho;       // header offset in a staged entry
+  // fixed-point accumulation (FP32 grids): the pass's force and torque
+  // maxima as namax partial pairs from k_absmax, and per-entry bounds
+  // |contribution| <= K0 max|F| + K1 max|T| at any grid point
+  const float* amax;
+  int namax;
+  double K0, K1;
 };
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz);
+// partial maxima of |F| and |T| (3n values each; T may be null) into out[2 b], out[2 b + 1]
+constexpr int DPS_AMAX_BLOCKS = 64;
+template <class Tin> void launch_absmax(const void* F, const void* Tq, int n3, float* out, hipStream_t s);
 
 // Spreading entry lists of one pass, built from the located particles: every
 // (particle, kind) entry is listed in each (tile, z chunk) cell its support

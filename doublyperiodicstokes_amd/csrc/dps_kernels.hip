@@ -22,6 +22,7 @@
 // [c][z or k][ky][kx] with kx in [0, Nx/2] (r2c half spectrum).
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 #include "dps_core.h"
 #include "dps_internal.h"
 
@@ -458,11 +459,21 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   // at ~193 cycles per 64-lane instruction per CU, an FP64 one (ds_add_f64)
   // at ~9 (measured), so FP32 weights are summed in FP64 and rounded once on
   // the way out
+  //
+  // With FP32 grids the tile is int32 fixed point instead: ds_add_u32 is the
+  // fastest LDS atomic (~6 cycles), the tile takes half the LDS (more
+  // workgroups per CU), and integer sums do not depend on the order of the
+  // additions, so the spread grid is bitwise reproducible.  The scale 2^S
+  // keeps every partial sum below 2^30: a point receives at most one term
+  // per entry of the cell, each at most K0 max|F| + K1 max|T|.
+  constexpr bool FX = sizeof(T) == 4;
+  typedef typename std::conditional<FX, int, double>::type TA;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  double* acc = reinterpret_cast<double*>(smem_raw);
+  TA* acc = reinterpret_cast<TA*>(smem_raw);
   const int vol = A.ZC * PL;
   T* bst = reinterpret_cast<T*>(acc + 3 * vol);           // [B][SLOT] staged entries
   int* queue = reinterpret_cast<int*>(bst + B * SLOT);     // p * 2 + kind
+  float* scl = reinterpret_cast<float*>(queue + 2 * 256);  // fixed-point scale and its inverse
   T* __restrict__ grid = reinterpret_cast<T*>(A.grid);
   // cells in list order, consecutive cells (the chunks of a tile, then the
   // next tile) on one XCD, whose L2 then holds their shared records
@@ -471,15 +482,43 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int X0 = tx * TX, Y0 = ty * TY, Z0 = zc * A.ZC;
   const int nX = min(TX, A.Nx - X0), nY = min(TY, A.Ny - Y0), nZ = min(A.ZC, A.Nz - Z0);
-  for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < 3 * vol; i += blockDim.x) acc[i] = (TA)0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwaves = blockDim.x >> 6;
   // this cell's entries, listed when the positions were set
   const int e0 = A.lstart[cell];
   const int ncand = max(0, min(A.lstart[cell + 1], A.lcap) - e0);   // within the list's capacity
-  // the zeroed accumulator is complete before any thread adds to it or, for
-  // a cell without entries, writes it out
+  if (FX && wave == 0) {
+    // the scale: a NaN in the inputs propagates to the whole tile
+    float fm = 0.0f, tm = 0.0f;
+    for (int i = lane; i < A.namax; i += 64) {
+      const float a = A.amax[2 * i], b = A.amax[2 * i + 1];
+      fm = (a > fm || a != a) ? a : fm;
+      tm = (b > tm || b != b) ? b : tm;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const float a = __shfl_xor(fm, o, 64), b = __shfl_xor(tm, o, 64);
+      fm = (a > fm || a != a) ? a : fm;
+      tm = (b > tm || b != b) ? b : tm;
+    }
+    if (lane == 0) {
+      const double bound = (double)ncand * (A.K0 * fm + A.K1 * tm);
+      int e = 0;
+      (void)frexp(bound, &e);   // bound < 2^e
+      const int S = min(30 - e, 120);
+      scl[0] = ldexpf(1.0f, S);
+      scl[1] = isfinite(bound) ? ldexpf(1.0f, -S) : __builtin_nanf("");
+    }
+  }
+  // the zeroed accumulator and the scale are complete before any thread adds
+  // to the tile or, for a cell without entries, writes it out
   __syncthreads();
+  const float sc = FX ? scl[0] : 1.0f;
+  auto add = [&](TA* d, T v) {
+    if constexpr (FX) atomicAdd(d, __float2int_rn((float)v * sc));
+    else atomicAdd(d, (double)v);
+  };
   // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2; paired
   // entries: force support, torque support, force, torque
   const int HO = A.ho;
@@ -602,7 +641,7 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
               if (phase >= P || col >= L) continue;
               const int b = col / cx, a = col - b * cx;
               const int i = ix0 + a, j = iy0 + b;
-              double* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
+              TA* dcol = acc + (kls - Z0) * PL + (ly0 + b) * RS + (lx0 + a);   // row kls + q at + q PL
               const int q1 = qb2;
               if (kind == 2) {
                 // force (monopole) and torque curl at each point, one atomic
@@ -626,10 +665,10 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                   }
                   const T w = wm * zm;
                   const T gx = ax * zA, gy = ay * zA, gz = az * zB;
-                  double* d = dcol + q * PL;
-                  atomicAdd(d, (double)(w * f0 + h * (gy * t2 - gz * t1)));
-                  atomicAdd(d + vol, (double)(w * f1 + h * (gz * t0 - gx * t2)));
-                  atomicAdd(d + 2 * vol, (double)(w * f2 + h * (gx * t1 - gy * t0)));
+                  TA* d = dcol + q * PL;
+                  add(d, (w * f0 + h * (gy * t2 - gz * t1)));
+                  add(d + vol, (w * f1 + h * (gz * t0 - gx * t2)));
+                  add(d + 2 * vol, (w * f2 + h * (gx * t1 - gy * t0)));
                 }
               } else if (kind == 0) {
                 const T wxy = wxr[i] * wyr[j];
@@ -638,24 +677,24 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                 for (; q + 3 * P < q1; q += 4 * P) {
                   const T z0 = zr[q], z1 = zr[q + P], z2 = zr[q + 2 * P], z3 = zr[q + 3 * P];
                   const T w0 = wxy * z0, w1 = wxy * z1, w2 = wxy * z2, w3 = wxy * z3;
-                  double* d0 = dcol + q * PL;
-                  atomicAdd(d0, (double)(w0 * g0)); atomicAdd(d0 + vol, (double)(w0 * g1));
-                  atomicAdd(d0 + 2 * vol, (double)(w0 * g2));
-                  double* d1 = d0 + P * PL;
-                  atomicAdd(d1, (double)(w1 * g0)); atomicAdd(d1 + vol, (double)(w1 * g1));
-                  atomicAdd(d1 + 2 * vol, (double)(w1 * g2));
-                  double* d2 = d1 + P * PL;
-                  atomicAdd(d2, (double)(w2 * g0)); atomicAdd(d2 + vol, (double)(w2 * g1));
-                  atomicAdd(d2 + 2 * vol, (double)(w2 * g2));
-                  double* d3 = d2 + P * PL;
-                  atomicAdd(d3, (double)(w3 * g0)); atomicAdd(d3 + vol, (double)(w3 * g1));
-                  atomicAdd(d3 + 2 * vol, (double)(w3 * g2));
+                  TA* d0 = dcol + q * PL;
+                  add(d0, (w0 * g0)); add(d0 + vol, (w0 * g1));
+                  add(d0 + 2 * vol, (w0 * g2));
+                  TA* d1 = d0 + P * PL;
+                  add(d1, (w1 * g0)); add(d1 + vol, (w1 * g1));
+                  add(d1 + 2 * vol, (w1 * g2));
+                  TA* d2 = d1 + P * PL;
+                  add(d2, (w2 * g0)); add(d2 + vol, (w2 * g1));
+                  add(d2 + 2 * vol, (w2 * g2));
+                  TA* d3 = d2 + P * PL;
+                  add(d3, (w3 * g0)); add(d3 + vol, (w3 * g1));
+                  add(d3 + 2 * vol, (w3 * g2));
                 }
                 for (; q < q1; q += P) {
                   const T w = wxy * zr[q];
-                  double* d = dcol + q * PL;
-                  atomicAdd(d, (double)(w * g0)); atomicAdd(d + vol, (double)(w * g1));
-                  atomicAdd(d + 2 * vol, (double)(w * g2));
+                  TA* d = dcol + q * PL;
+                  add(d, (w * g0)); add(d + vol, (w * g1));
+                  add(d + 2 * vol, (w * g2));
                 }
               } else {
                 const T h = (T)0.5;
@@ -665,26 +704,26 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                   const T zA0 = zr[q], zB0 = zr[W + q], zA1 = zr[q + P], zB1 = zr[W + q + P];
                   {
                     const T gx = ax * zA0, gy = ay * zA0, gz = az * zB0;
-                    double* d = dcol + q * PL;
-                    atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
-                    atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
-                    atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
+                    TA* d = dcol + q * PL;
+                    add(d, (h * (gy * f2 - gz * f1)));
+                    add(d + vol, (h * (gz * f0 - gx * f2)));
+                    add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
                   }
                   {
                     const T gx = ax * zA1, gy = ay * zA1, gz = az * zB1;
-                    double* d = dcol + (q + P) * PL;
-                    atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
-                    atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
-                    atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
+                    TA* d = dcol + (q + P) * PL;
+                    add(d, (h * (gy * f2 - gz * f1)));
+                    add(d + vol, (h * (gz * f0 - gx * f2)));
+                    add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
                   }
                 }
                 for (; q < q1; q += P) {
                   const T zA = zr[q], zB = zr[W + q];
                   const T gx = ax * zA, gy = ay * zA, gz = az * zB;
-                  double* d = dcol + q * PL;
-                  atomicAdd(d, (double)(h * (gy * f2 - gz * f1)));
-                  atomicAdd(d + vol, (double)(h * (gz * f0 - gx * f2)));
-                  atomicAdd(d + 2 * vol, (double)(h * (gx * f1 - gy * f0)));
+                  TA* d = dcol + q * PL;
+                  add(d, (h * (gy * f2 - gz * f1)));
+                  add(d + vol, (h * (gz * f0 - gx * f2)));
+                  add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
                 }
               }
             }
@@ -703,10 +742,12 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
     if (ix < nX)
       for (int c = 0; c < 3; ++c) {
         T* gc = grid + ((size_t)c * A.Nz + Z0) * A.Ny * A.Nx + (size_t)Y0 * A.Nx + X0 + ix;
-        const double* ac = acc + c * vol + ix;
+        const TA* ac = acc + c * vol + ix;
+        const float isc = FX ? scl[1] : 1.0f;
         for (int row = threadIdx.x / TX; row < nrow; row += rstep) {
           const int kz = row / TY, jy = row % TY;
-          if (jy < nY) gc[((size_t)kz * A.Ny + jy) * A.Nx] = (T)ac[kz * PL + jy * RS];
+          if (jy < nY) gc[((size_t)kz * A.Ny + jy) * A.Nx] = FX ? (T)((float)ac[kz * PL + jy * RS] * isc)
+                                                                 : (T)ac[kz * PL + jy * RS];
         }
       }
   }
@@ -1716,6 +1757,50 @@ void launch_sort(int* key, int* counts, int* rank, int* start, int* order, int N
                        order, N);
 }
 
+// partial maxima of |F| and |T|; NaN propagates
+template <class Tin>
+__global__ void __launch_bounds__(256) k_absmax(const Tin* __restrict__ F, const Tin* __restrict__ Tq, int n3,
+                                                float* __restrict__ out)
+{
+  __shared__ float red[2][4];
+  float fm = 0.0f, tm = 0.0f;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n3; i += gridDim.x * 256) {
+    const float a = fabsf((float)F[i]);
+    fm = (a > fm || a != a) ? a : fm;
+    if (Tq) {
+      const float b = fabsf((float)Tq[i]);
+      tm = (b > tm || b != b) ? b : tm;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const float a = __shfl_xor(fm, o, 64), b = __shfl_xor(tm, o, 64);
+    fm = (a > fm || a != a) ? a : fm;
+    tm = (b > tm || b != b) ? b : tm;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = fm;
+    red[1][threadIdx.x >> 6] = tm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      fm = (red[0][w] > fm || red[0][w] != red[0][w]) ? red[0][w] : fm;
+      tm = (red[1][w] > tm || red[1][w] != red[1][w]) ? red[1][w] : tm;
+    }
+    out[2 * blockIdx.x] = fm;
+    out[2 * blockIdx.x + 1] = tm;
+  }
+}
+
+template <class Tin>
+void launch_absmax(const void* F, const void* Tq, int n3, float* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_absmax<Tin>, dim3(DPS_AMAX_BLOCKS), dim3(256), 0, s, (const Tin*)F, (const Tin*)Tq, n3, out);
+}
+template void launch_absmax<float>(const void*, const void*, int, float*, hipStream_t);
+template void launch_absmax<double>(const void*, const void*, int, float*, hipStream_t);
+
 void launch_lists(const ListArgs& A, hipStream_t s)
 {
   const int ncells = A.ntx * A.nty * A.nch;
@@ -1738,9 +1823,10 @@ int spread_npf(int rs_max)
 
 size_t spread_lds_bytes(const SpreadArgs& A, size_t tsz)
 {
-  // FP64 accumulator, staged batch, queue
-  return (size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) * sizeof(double) +
-         (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max) * tsz + 2 * 256 * sizeof(int);
+  // accumulator (int32 fixed point with FP32 grids, else FP64), staged
+  // batch, queue, scale
+  return (size_t)3 * A.ZC * (SP_TY * (SP_TX + 1) + 1) * (tsz == 4 ? 4 : 8) +
+         (size_t)DPS_SPREAD_BATCH * 16 * spread_npf(A.rs_max) * tsz + 2 * 256 * sizeof(int) + 4 * sizeof(float);
 }
 
 template <class T, class Tin>
