@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 #include "../../include/dpstokes_hip.h"
 #include "dps_internal.h"
@@ -161,6 +162,8 @@ struct dps_handle {
   int sp_lpm = 0, sp_e = 0, sp_G = 0;
   void *d_TfP = nullptr, *d_TiP = nullptr;
   double *d_fib = nullptr, *d_fab = nullptr, *d_fG = nullptr;
+  int* d_fidx = nullptr;   // [Nm] factor-table entry of each mode
+  int n_fac = 0;           // distinct |k|^2 (factor-table entries)
 
   rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
   bool fft_y = false;        // rocFFT runs the x pass only; k_fft_y the y pass
@@ -622,23 +625,49 @@ int allocate_fused(dps_handle* h)
     if ((rc = h->upload((double**)&h->d_TfP, tf))) return rc;
     if ((rc = h->upload((double**)&h->d_TiP, ti))) return rc;
   }
-  const size_t nf = (size_t)h->Nm * E * LPM;
+  // The factorisation depends on |k|^2 only: one table entry per distinct
+  // value (bit for bit, as ksq_of forms it in the kernels), shared by the
+  // modes (+-kx, +-ky) and, on a square box, (ky, kx) -- about a quarter of
+  // the modes on a square grid.
+  std::vector<int> fidx(h->Nm);
+  std::vector<double> kq;
+  {
+    std::unordered_map<uint64_t, int> seen;
+    seen.reserve((size_t)h->Nm / 2);
+    for (int m = 0; m < h->Nm; ++m) {
+      const int iy = m / h->Nkx, ix = m - iy * h->Nkx;
+      const double k2 = ksq_of(kvec(ix, h->Nx, h->P.Lx), kvec(iy, h->Ny, h->P.Ly));
+      uint64_t bits;
+      std::memcpy(&bits, &k2, sizeof bits);
+      auto it = seen.emplace(bits, (int)kq.size());
+      if (it.second) kq.push_back(k2);
+      fidx[m] = it.first->second;
+    }
+  }
+  const int nk = (int)kq.size();
+  h->n_fac = nk;
+  if ((rc = h->upload(&h->d_fidx, fidx))) return rc;
+  const size_t nf = (size_t)nk * E * LPM;
   if ((rc = h->alloc(&h->d_fib, nf))) return rc;
   if ((rc = h->alloc(&h->d_fab, nf))) return rc;
-  if ((rc = h->alloc(&h->d_fG, (size_t)h->Nm * 4))) return rc;
-  double *t_ib = nullptr, *t_ab = nullptr;
-  if (hipMalloc((void**)&t_ib, (size_t)(Nz + 1) * h->Nm * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&t_ab, (size_t)Nz * h->Nm * sizeof(double)) != hipSuccess) {
+  if ((rc = h->alloc(&h->d_fG, (size_t)nk * 4))) return rc;
+  double *t_ib = nullptr, *t_ab = nullptr, *t_kq = nullptr;
+  if (hipMalloc((void**)&t_ib, (size_t)(Nz + 1) * nk * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&t_ab, (size_t)Nz * nk * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&t_kq, (size_t)nk * sizeof(double)) != hipSuccess) {
     if (t_ib) (void)hipFree(t_ib);
+    if (t_ab) (void)hipFree(t_ab);
     return fail(DPS_ERR_ALLOC, "hipMalloc of the factorisation scratch failed");
   }
   const BvpArgs A = bvp_args(h);
-  launch_bvp_factor(A.B, h->Nx, h->Ny, h->Nkx, h->Nm, h->P.Lx, h->P.Ly, t_ib, t_ab, LPM, E, h->d_fib,
-                    h->d_fab, h->d_fG, 0);
-  hipError_t e1 = hipGetLastError();
+  hipError_t e0 = hipMemcpy(t_kq, kq.data(), (size_t)nk * sizeof(double), hipMemcpyHostToDevice);
+  if (e0 == hipSuccess)
+    launch_bvp_factor(A.B, t_kq, nk, t_ib, t_ab, LPM, E, h->d_fib, h->d_fab, h->d_fG, 0);
+  hipError_t e1 = (e0 != hipSuccess) ? e0 : hipGetLastError();
   hipError_t e2 = hipStreamSynchronize(0);
   (void)hipFree(t_ib);
   (void)hipFree(t_ab);
+  (void)hipFree(t_kq);
   if (e1 != hipSuccess || e2 != hipSuccess) return fail(DPS_ERR_HIP, "factorisation kernel failed");
   return DPS_OK;
 }
@@ -913,7 +942,7 @@ SpecArgs spec_args(dps_handle* h)
   A.NH = h->ct.NH;
   A.lpm = h->sp_lpm; A.e = h->sp_e; A.G = h->sp_G;
   A.TfP = h->d_TfP; A.TiP = h->d_TiP;
-  A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
+  A.fac_idx = h->d_fidx; A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
   A.kxs = h->d_kxs; A.kys = h->d_kys;
   A.zpts = h->d_zpts;
   A.Hs = h->Lz;

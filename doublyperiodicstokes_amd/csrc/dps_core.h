@@ -18,6 +18,16 @@
 
 namespace dps {
 
+// |k|^2 of a planar wavenumber, one rounding (fma), identical bit for bit
+// under a swap of kx and ky and under sign changes: modes that share |k| on a
+// square box share one factorisation of their BVP.
+DPS_HD double ksq_of(double kx, double ky)
+{
+  const double a = fabs(kx), b = fabs(ky);
+  const double hi = fmax(a, b), lo = fmin(a, b);
+  return fma(hi, hi, lo * lo);
+}
+
 // ---------------------------------------------------------------------------
 // minimal FP64 complex
 // ---------------------------------------------------------------------------
@@ -339,7 +349,7 @@ DPS_HD void bvp_mode(const BvpConsts& B, double kx, double ky, int nyq_x, int ny
   const int Nz = B.Nz;
   const long st = sc.stride;
   const double H = B.H;
-  const double Ksq = kx * kx + ky * ky;
+  const double Ksq = ksq_of(kx, ky);
   const double K = sqrt(Ksq);
   const double H2K = (H * H) * K;
   const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
@@ -676,8 +686,11 @@ DPS_HD void row_eliminate(const Row4& x, Row4& y, int col, double inv)
 }
 
 // Gaussian elimination with partial pivoting (largest |entry| of the column,
-// first one on ties) and back substitution; x receives the solution.
-DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx* x)
+// first one on ties) and back substitution; (x0, x1, x2, x3) receive the
+// solution (named outputs: an output array whose element a branch picks is
+// kept in private memory on the GPU).
+DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx& x0, cplx& x1, cplx& x2,
+                         cplx& x3)
 {
   // column 0
   {
@@ -713,66 +726,120 @@ DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx* x)
     const double inv = 1.0 / R2.a2;
     row_eliminate(R2, R3, 2, inv);
   }
-  x[3] = R3.b / R3.a3;
-  x[2] = (R2.b - R2.a3 * x[3]) / R2.a2;
-  x[1] = ((R1.b - R1.a2 * x[2]) - R1.a3 * x[3]) / R1.a1;
-  x[0] = (((R0.b - R0.a1 * x[1]) - R0.a2 * x[2]) - R0.a3 * x[3]) / R0.a0;
+  x3 = R3.b / R3.a3;
+  x2 = (R2.b - R2.a3 * x3) / R2.a2;
+  x1 = ((R1.b - R1.a2 * x2) - R1.a3 * x3) / R1.a1;
+  x0 = (((R0.b - R0.a1 * x1) - R0.a2 * x2) - R0.a3 * x3) / R0.a0;
 }
 
+// Wall-correction coefficients of the bottom wall (DPTools.cpp:617-633),
+// basis (e^{-Kz}, z e^{-Kz}): c[comp][basis].
+DPS_HD void bottom_coeffs(double kx, double ky, const cplx* wall_bot, cplx& c00, cplx& c01, cplx& c10,
+                          cplx& c11, cplx& c20, cplx& c21)
+{
+  const double K = sqrt(ksq_of(kx, ky));
+  const cplx fx = -wall_bot[0], fy = -wall_bot[1], fz = -wall_bot[2];
+  const cplx alpha = kx * fx + ky * fy + times_i(K * fz);
+  c00 = fx; c01 = ((-kx) * alpha) / K;
+  c10 = fy; c11 = ((-ky) * alpha) / K;
+  c20 = fz; c21 = -times_i(alpha);
+}
+
+// Wall-correction unknowns of the slit (DPTools.cpp:727-812), basis
+// (e^{-kz}, e^{k(z-Hs)}, z e^{-kz}, z e^{k(z-Hs)}): X0..X7 of the 8x8 system.
+DPS_HD void slit_unknowns(const BvpConsts& B, double kx, double ky, const cplx* wall_bot, const cplx* wall_top,
+                          cplx& X0, cplx& X1, cplx& X2, cplx& X3, cplx& X4, cplx& X5, cplx& X6, cplx& X7)
+{
+  const double k = sqrt(ksq_of(kx, ky));
+  const double Hs = B.Hs;
+  const double fac = 1.0 / 2.0 / B.eta;
+  const double enkh = exp(-k * Hs);
+  const cplx fbx = -wall_bot[0], fby = -wall_bot[1], fbz = -wall_bot[2];
+  const cplx ftx = -wall_top[0], fty = -wall_top[1], ftz = -wall_top[2];
+  // right-hand side of the 8x8 system
+  const cplx rhs0 = times_i(-kx * fbx) - times_i(ky * fby);
+  const cplx rhs1 = times_i(-kx * ftx) - times_i(ky * fty);
+  const cplx rhs2 = fbx, rhs3 = fby, rhs4 = fbz;
+  const cplx rhs5 = ftx, rhs6 = fty, rhs7 = ftz;
+  // rows 0,1,4,7 of the 8x8 system couple only (x0, x1, x6, x7) and are
+  // real: solve that 4x4 block with partial pivoting, then the two 2x2
+  // blocks for (x2, x3) and (x4, x5).
+  Row4 R0 = {fac, enkh * fac, -k, k * enkh, rhs0};
+  Row4 R1 = {(1 - k * Hs) * enkh * fac, (1 + k * Hs) * fac, -k * enkh, k, rhs1};
+  Row4 R2 = {0, 0, 1, enkh, rhs4};
+  Row4 R3 = {Hs * enkh * fac, Hs * fac, enkh, 1, rhs7};
+  solve4_pivot(R0, R1, R2, R3, X0, X1, X6, X7);
+  // rows 2,5: x2 + enkh x3 = b2 ; enkh x2 + x3 = b5 - (A50 X0 + A51 X1)
+  const double d2 = 1 - enkh * enkh;
+  const cplx r5 = rhs5 - (times_i(-kx * Hs * enkh * fac / k * X0) + times_i(kx * Hs * fac / k * X1));
+  X2 = (rhs2 - enkh * r5) / d2;
+  X3 = (r5 - enkh * rhs2) / d2;
+  const cplx r6 = rhs6 - (times_i(-ky * Hs * enkh * fac / k * X0) + times_i(ky * Hs * fac / k * X1));
+  X4 = (rhs3 - enkh * r6) / d2;
+  X5 = (r6 - enkh * rhs3) / d2;
+}
+
+// Slit coefficients c[comp][basis] from the unknowns.
+DPS_HD void slit_coeffs(const BvpConsts& B, double kx, double ky, const cplx* wall_bot, const cplx* wall_top,
+                        cplx& c00, cplx& c01, cplx& c02, cplx& c03, cplx& c10, cplx& c11, cplx& c12,
+                        cplx& c13, cplx& c20, cplx& c21, cplx& c22, cplx& c23)
+{
+  cplx X0, X1, X2, X3, X4, X5, X6, X7;
+  slit_unknowns(B, kx, ky, wall_bot, wall_top, X0, X1, X2, X3, X4, X5, X6, X7);
+  const double k = sqrt(ksq_of(kx, ky));
+  const double fac = 1.0 / 2.0 / B.eta;
+  c00 = X2; c01 = X3;
+  c02 = times_i(-(kx * fac / k) * X0); c03 = times_i((kx * fac / k) * X1);
+  c10 = X4; c11 = X5;
+  c12 = times_i(-(ky * fac / k) * X0); c13 = times_i((ky * fac / k) * X1);
+  c20 = X6; c21 = X7;
+  c22 = fac * X0; c23 = fac * X1;
+}
+
+// Wall correction of one mode into res: zeroed, then the entries of the
+// geometry's basis written on its branch.
 DPS_HD void wall_correction(const BvpConsts& B, double kx, double ky, const cplx* wall_bot,
                             const cplx* wall_top, ModeResult& res)
 {
-  const double K = sqrt(kx * kx + ky * ky);
   for (int c = 0; c < 3; ++c)
     for (int b = 0; b < 4; ++b) res.corr[c][b] = mk(0, 0);
   res.nbasis = 0;
   if (B.geometry == MODE_BOTTOM) {
-    // DPTools.cpp:617-633; basis (e^{-Kz}, z e^{-Kz})
-    const cplx fx = -wall_bot[0], fy = -wall_bot[1], fz = -wall_bot[2];
-    const cplx alpha = kx * fx + ky * fy + times_i(K * fz);
     res.nbasis = 2;
-    res.corr[0][0] = fx; res.corr[0][1] = ((-kx) * alpha) / K;
-    res.corr[1][0] = fy; res.corr[1][1] = ((-ky) * alpha) / K;
-    res.corr[2][0] = fz; res.corr[2][1] = -times_i(alpha);
+    bottom_coeffs(kx, ky, wall_bot, res.corr[0][0], res.corr[0][1], res.corr[1][0], res.corr[1][1],
+                  res.corr[2][0], res.corr[2][1]);
   } else if (B.geometry == MODE_SLIT) {
-    // DPTools.cpp:727-812; basis (e^{-kz}, e^{k(z-Hs)}, z e^{-kz}, z e^{k(z-Hs)})
-    const double k = sqrt(kx * kx + ky * ky);
-    const double Hs = B.Hs;
-    const double fac = 1.0 / 2.0 / B.eta;
-    const double enkh = exp(-k * Hs);
-    const cplx fbx = -wall_bot[0], fby = -wall_bot[1], fbz = -wall_bot[2];
-    const cplx ftx = -wall_top[0], fty = -wall_top[1], ftz = -wall_top[2];
-    cplx rhs[8];
-    rhs[0] = times_i(-kx * fbx) - times_i(ky * fby);
-    rhs[1] = times_i(-kx * ftx) - times_i(ky * fty);
-    rhs[2] = fbx; rhs[3] = fby; rhs[4] = fbz;
-    rhs[5] = ftx; rhs[6] = fty; rhs[7] = ftz;
-    // rows 0,1,4,7 of the 8x8 system couple only (x0, x1, x6, x7) and are
-    // real: solve that 4x4 block with partial pivoting, then the two 2x2
-    // blocks for (x2, x3) and (x4, x5).
-    Row4 R0 = {fac, enkh * fac, -k, k * enkh, rhs[0]};
-    Row4 R1 = {(1 - k * Hs) * enkh * fac, (1 + k * Hs) * fac, -k * enkh, k, rhs[1]};
-    Row4 R2 = {0, 0, 1, enkh, rhs[4]};
-    Row4 R3 = {Hs * enkh * fac, Hs * fac, enkh, 1, rhs[7]};
-    cplx xs4[4];
-    solve4_pivot(R0, R1, R2, R3, xs4);
-    const cplx X0 = xs4[0], X1 = xs4[1], X6 = xs4[2], X7 = xs4[3];
-    // rows 2,5: x2 + enkh x3 = b2 ; enkh x2 + x3 = b5 - (A50 X0 + A51 X1)
-    const double d2 = 1 - enkh * enkh;
-    const cplx r5 = rhs[5] - (times_i(-kx * Hs * enkh * fac / k * X0) + times_i(kx * Hs * fac / k * X1));
-    const cplx X2 = (rhs[2] - enkh * r5) / d2;
-    const cplx X3 = (r5 - enkh * rhs[2]) / d2;
-    const cplx r6 = rhs[6] - (times_i(-ky * Hs * enkh * fac / k * X0) + times_i(ky * Hs * fac / k * X1));
-    const cplx X4 = (rhs[3] - enkh * r6) / d2;
-    const cplx X5 = (r6 - enkh * rhs[3]) / d2;
     res.nbasis = 4;
-    res.corr[0][0] = X2; res.corr[0][1] = X3;
-    res.corr[0][2] = times_i(-(kx * fac / k) * X0); res.corr[0][3] = times_i((kx * fac / k) * X1);
-    res.corr[1][0] = X4; res.corr[1][1] = X5;
-    res.corr[1][2] = times_i(-(ky * fac / k) * X0); res.corr[1][3] = times_i((ky * fac / k) * X1);
-    res.corr[2][0] = X6; res.corr[2][1] = X7;
-    res.corr[2][2] = fac * X0; res.corr[2][3] = fac * X1;
+    slit_coeffs(B, kx, ky, wall_bot, wall_top, res.corr[0][0], res.corr[0][1], res.corr[0][2], res.corr[0][3],
+                res.corr[1][0], res.corr[1][1], res.corr[1][2], res.corr[1][3], res.corr[2][0],
+                res.corr[2][1], res.corr[2][2], res.corr[2][3]);
   }
+}
+
+// The same into named scalars, every entry of res assigned once at the end.
+// In a GPU kernel res then stays in registers; the form above keeps it in
+// private memory (the branches write different entries).  The FP32 spectral
+// kernel takes this form (no private-memory traffic, and faster); the FP64
+// one the form above, which measures faster there.
+DPS_HD void wall_correction_regs(const BvpConsts& B, double kx, double ky, const cplx* wall_bot,
+                                 const cplx* wall_top, ModeResult& res)
+{
+  const cplx z0 = mk(0, 0);
+  cplx c00 = z0, c01 = z0, c02 = z0, c03 = z0;
+  cplx c10 = z0, c11 = z0, c12 = z0, c13 = z0;
+  cplx c20 = z0, c21 = z0, c22 = z0, c23 = z0;
+  int nb = 0;
+  if (B.geometry == MODE_BOTTOM) {
+    nb = 2;
+    bottom_coeffs(kx, ky, wall_bot, c00, c01, c10, c11, c20, c21);
+  } else if (B.geometry == MODE_SLIT) {
+    nb = 4;
+    slit_coeffs(B, kx, ky, wall_bot, wall_top, c00, c01, c02, c03, c10, c11, c12, c13, c20, c21, c22, c23);
+  }
+  res.nbasis = nb;
+  res.corr[0][0] = c00; res.corr[0][1] = c01; res.corr[0][2] = c02; res.corr[0][3] = c03;
+  res.corr[1][0] = c10; res.corr[1][1] = c11; res.corr[1][2] = c12; res.corr[1][3] = c13;
+  res.corr[2][0] = c20; res.corr[2][1] = c21; res.corr[2][2] = c22; res.corr[2][3] = c23;
 }
 
 // k = 0 mode for the walled geometries (Solvers.py:482-516, 563-599): the

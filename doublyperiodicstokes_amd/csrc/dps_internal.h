@@ -228,9 +228,10 @@ struct SpecArgs {
   int lpm, e, G;          // lanes per mode, rows per lane, modes per block
   const void* TfP;        // [NH][LPM][E] forward rows in lane order (grid precision)
   const void* TiP;        // [ceil(Nz/2)][LPM][E] inverse rows in lane order (scaled by 1/(Nx Ny))
-  const double* fac_ib;   // [Nm][LPM][E] reciprocal pivots of each lane's rows
-  const double* fac_ab;   // [Nm][LPM][E] A^{-1}B of each lane's rows
-  const double* fac_G;    // [Nm][4] inverse Schur complement
+  const int* fac_idx;     // [Nm] entry of each mode in the factor tables (its |k|^2)
+  const double* fac_ib;   // [nk][LPM][E] reciprocal pivots of each lane's rows
+  const double* fac_ab;   // [nk][LPM][E] A^{-1}B of each lane's rows
+  const double* fac_G;    // [nk][4] inverse Schur complement
   const double *kxs, *kys;   // grid wavenumbers by index
   const double* zpts;
   double Hs, inv_nxy;
@@ -260,9 +261,8 @@ constexpr int spectral_waves_per_simd(int lpm, int e, bool mixed)
   const int w = blocks * (G * lpm / 64) / 4;
   return w < 1 ? 1 : (w > 3 ? 3 : w);
 }
-void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
-                       double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,
-                       hipStream_t s);
+void launch_bvp_factor(const BvpConsts& B, const double* ksq, int nk, double* s_ib, double* s_ab, int lpm,
+                       int e, double* fib, double* fab, double* fG, hipStream_t s);
 template <class T> void launch_spectral(const SpecArgs& A, const void* in, void* out, hipStream_t s);
 template <class T> void launch_ref_to_grid(const double* ref, void* g, int Nx, int Ny, int Nz, hipStream_t s);
 template <class T> void launch_grid_to_ref(const void* g, double* ref, int Nx, int Ny, int Nz, hipStream_t s);

@@ -1269,7 +1269,7 @@ __global__ void __launch_bounds__(256) k_zinv(ZArgs A, const typename C2<T>::t* 
     for (int b = 0; b < 4; ++b) { cc[b].re = cr[b].x; cc[b].im = cr[b].y; }
     const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
     const double kx = A.kxs[ix], ky = A.kys[iy];
-    K = sqrt(kx * kx + ky * ky);
+    K = sqrt(ksq_of(kx, ky));
     if (m == 0) nb = 0;
   }
   V* dst = out + (size_t)c * N * A.Nm;

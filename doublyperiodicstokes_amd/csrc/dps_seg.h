@@ -268,14 +268,16 @@ DPS_HD cplx seg_xrow(int row, int Nz, cplx x, double ab, cplx Y, cplx Y0, cplx Y
 // forcing, then read back through fin by their solves, so the pressure
 // coefficients hold no registers during the velocity solves.  fin and fout
 // must then address the same storage, and fout must store what it is given.
-template <int LPM, int E, int NRV = 3, bool STAGE = false, class L, class FIn, class FOut>
+// RREG: the wall correction is formed by wall_correction_regs (res held in
+// registers) rather than wall_correction; the results are identical.
+template <int LPM, int E, int NRV = 3, bool STAGE = false, bool RREG = false, class L, class FIn, class FOut>
 DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, int nyq_x, int nyq_y,
                          const double (&ib)[E], const double (&ab)[E], const double* Gi, const FIn& fin,
                          FOut& fout, ModeResult& res, bool avg = false, double kxa = 0, double kya = 0)
 {
   const int Nz = B.Nz;
   const double H = B.H;
-  const double Ksq = kx * kx + ky * ky;
+  const double Ksq = ksq_of(kx, ky);
   const double K = sqrt(Ksq);
   const double H2K = (H * H) * K;
   const cplx Dx = nyq_x ? mk(0, 0) : mk(0, kx);
@@ -482,10 +484,12 @@ DPS_HD void seg_bvp_mode(const BvpConsts& B, const L& ln, double kx, double ky, 
   }
 #undef C0
 #undef C1
-  wall_correction(B, kx, ky, wall_bot, wall_top, res);
+  if constexpr (RREG) wall_correction_regs(B, kx, ky, wall_bot, wall_top, res);
+  else wall_correction(B, kx, ky, wall_bot, wall_top, res);
   if (avg) {
     ModeResult r2;
-    wall_correction(B, kxa, kya, wall_bot, wall_top, r2);
+    if constexpr (RREG) wall_correction_regs(B, kxa, kya, wall_bot, wall_top, r2);
+    else wall_correction(B, kxa, kya, wall_bot, wall_top, r2);
     #pragma unroll
     for (int c = 0; c < 3; ++c)
       #pragma unroll

@@ -161,6 +161,8 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   const int m0 = xcd_group(blockIdx.x, gridDim.x) * G;
   const int m = (m0 + g < Nm) ? m0 + g : -1;
   const int mc = (m0 + gc < Nm) ? m0 + gc : -1;
+  // factor-table entry of this lane group's mode, loaded ahead of phase 2
+  const int fu = (m > 0) ? A.fac_idx[m] : 0;
 
   // ---- phase 0: row tables and columns in ----
   // every load of a thread is issued before the first LDS write, so the
@@ -291,11 +293,12 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
       seg_bvp_k0<LPM, E>(A.B, ln, fin, fout);   // unpadded global tables
     } else {
       const double kx = A.kxs[ix], ky = A.kys[iy];
-      K = sqrt(kx * kx + ky * ky);
-      // [Nm][LPM][E] factor rows of this lane, [Nm][4] inverse Schur complement
+      K = sqrt(ksq_of(kx, ky));
+      // [nk][LPM][E] factor rows of this lane, [nk][4] inverse Schur
+      // complement, entry fu of the mode's |k|^2
       double ib[E], ab[E], Gi[4];
-      const double2* pib = (const double2*)(A.fac_ib + ((size_t)m * LPM + l) * E);
-      const double2* pab = (const double2*)(A.fac_ab + ((size_t)m * LPM + l) * E);
+      const double2* pib = (const double2*)(A.fac_ib + ((size_t)fu * LPM + l) * E);
+      const double2* pab = (const double2*)(A.fac_ab + ((size_t)fu * LPM + l) * E);
 #pragma unroll
       for (int q = 0; q < E; q += 2) {
         const double2 u = pib[q / 2], w = pab[q / 2];
@@ -303,7 +306,7 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
         ab[q] = w.x; ab[q + 1] = w.y;
       }
       {
-        const double2* pg = (const double2*)(A.fac_G + (size_t)m * 4);
+        const double2* pg = (const double2*)(A.fac_G + (size_t)fu * 4);
         const double2 g0 = pg[0], g1 = pg[1];
         Gi[0] = g0.x; Gi[1] = g0.y; Gi[2] = g1.x; Gi[3] = g1.y;
       }
@@ -312,7 +315,7 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
       // the mirror mode's grid wavenumbers, negated: the mirror index of a
       // Nyquist index is itself, any other index maps to minus its wavenumber
       const double kxa = nyqx ? 0.0 - kx : kx, kya = nyqy ? 0.0 - ky : ky;
-      seg_bvp_mode<LPM, E, DPS_SPEC_NRV, true>(B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res,
+      seg_bvp_mode<LPM, E, DPS_SPEC_NRV, true, sizeof(T) == 4>(B, ln, kx, ky, nyqx, nyqy, ib, ab, Gi, fin, fout, res,
                                                 nyqx || nyqy, kxa, kya);
     }
     if (l == 0)
@@ -444,34 +447,30 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   }
 }
 
-// Static factorisation of every mode (depends on |k| only), one lane per
-// mode, written in the lane order of k_spectral: fib/fab [Nm][lpm][e],
-// fG [Nm][4].
-__global__ void k_bvp_factor(BvpConsts B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
-                             double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab,
-                             double* fG)
+// Static factorisation of the BVP (it depends on |k|^2 only), one lane per
+// distinct |k|^2, written in the lane order of k_spectral: fib/fab
+// [nk][lpm][e], fG [nk][4].  The k = 0 entry (ksq = 0) stays zero.
+__global__ void k_bvp_factor(BvpConsts B, const double* ksq, int nk, double* s_ib, double* s_ab, int lpm,
+                             int e, double* fib, double* fab, double* fG)
 {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= Nm) return;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nk) return;
   ScratchView sc;
-  sc.stride = Nm;
-  sc.ib = s_ib + m;
-  sc.ab = s_ab + m;
+  sc.stride = nk;
+  sc.ib = s_ib + u;
+  sc.ab = s_ab + u;
   sc.P = sc.CP = sc.DP = sc.U0 = sc.U1 = sc.U2 = nullptr;
   double Gi[4] = {0, 0, 0, 0};
-  if (m != 0) {
-    const int iy = m / Nkx, ix = m - iy * Nkx;
-    const double kx = kvec(ix, Nx, Lx), ky = kvec(iy, Ny, Ly);
-    bvp_factor(B, kx * kx + ky * ky, sc, Gi);
-  }
+  const double k2 = ksq[u];
+  if (k2 != 0) bvp_factor(B, k2, sc, Gi);
   const int Nz = B.Nz;
   for (int row = 0; row < lpm * e; ++row) {
-    const size_t i = (size_t)m * lpm * e + row;   // row = l e + q
-    const bool live = (m != 0) && (row < Nz);
-    fib[i] = live ? s_ib[(size_t)(row + 1) * Nm + m] : 0.0;
-    fab[i] = live ? s_ab[(size_t)row * Nm + m] : 0.0;
+    const size_t i = (size_t)u * lpm * e + row;   // row = l e + q
+    const bool live = (k2 != 0) && (row < Nz);
+    fib[i] = live ? s_ib[(size_t)(row + 1) * nk + u] : 0.0;
+    fab[i] = live ? s_ab[(size_t)row * nk + u] : 0.0;
   }
-  for (int i = 0; i < 4; ++i) fG[(size_t)m * 4 + i] = Gi[i];
+  for (int i = 0; i < 4; ++i) fG[(size_t)u * 4 + i] = Gi[i];
 }
 
 }  // namespace
@@ -487,12 +486,11 @@ bool spectral_shape(int Nz, int* lpm, int* e)
 
 static_assert(TB_N == 9, "spectral_lds_bytes counts 9 row tables");
 
-void launch_bvp_factor(const BvpConsts& B, int Nx, int Ny, int Nkx, int Nm, double Lx, double Ly,
-                       double* s_ib, double* s_ab, int lpm, int e, double* fib, double* fab, double* fG,
-                       hipStream_t s)
+void launch_bvp_factor(const BvpConsts& B, const double* ksq, int nk, double* s_ib, double* s_ab, int lpm,
+                       int e, double* fib, double* fab, double* fG, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_bvp_factor, dim3((Nm + 63) / 64), dim3(64), 0, s, B, Nx, Ny, Nkx, Nm, Lx, Ly,
-                     s_ib, s_ab, lpm, e, fib, fab, fG);
+  hipLaunchKernelGGL(k_bvp_factor, dim3((nk + 63) / 64), dim3(64), 0, s, B, ksq, nk, s_ib, s_ab, lpm, e,
+                     fib, fab, fG);
 }
 
 template <class T>

@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_zinv_mma(ZArgs A, const T* __restrict__
       for (int b = 0; b < 4; ++b) cc[b] = im ? cr[b].y : cr[b].x;
       const int iy = m / A.Nkx, ix = m - iy * A.Nkx;
       const double kx = A.kxs[ix], ky = A.kys[iy];
-      K = (T)sqrt(kx * kx + ky * ky);
+      K = (T)sqrt(ksq_of(kx, ky));
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
