@@ -514,10 +514,19 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
   // the zeroed accumulator and the scale are complete before any thread adds
   // to the tile or, for a cell without entries, writes it out
   __syncthreads();
-  const float sc = FX ? scl[0] : 1.0f;
+  // the scale is folded into each column's coefficients: add takes values
+  // already in fixed-point units
+  const T sc = FX ? (T)scl[0] : (T)1;
   auto add = [&](TA* d, T v) {
-    if constexpr (FX) atomicAdd(d, __float2int_rn((float)v * sc));
+    if constexpr (FX) atomicAdd(d, __float2int_rn((float)v));
     else atomicAdd(d, (double)v);
+  };
+  // z weight q of a row list of n entries, zero outside [0, n): the index
+  // is clamped so the LDS read stays inside the slot
+  auto zw = [&](const T* z, int q, int n) {
+    const bool in = (unsigned)q < (unsigned)n;
+    const T v = z[in ? q : 0];
+    return in ? v : (T)0;
   };
   // header: sup.x, sup.y, sup.z (klo), sup.w (span), f0, f1, f2; paired
   // entries: force support, torque support, force, torque
@@ -652,31 +661,30 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                 const T* dyd = dxd + wx;
                 const T* zAd = dyd + wy;
                 const T* zBd = zAd + A.kd.wzmax;
-                const T wm = wxr[i] * wyr[j];
-                const T ax = dxd[i] * wyd[j], ay = wxd[i] * dyd[j], az = wxd[i] * wyd[j];
-                const T h = (T)0.5;
+                // with w = wm zm and g = (ax zA, ay zA, az zB) the point
+                // receives w f + 1/2 g x t: per component
+                //   P_c zm + Q_c zA + R_c zB  (R_2 = 0)
+                // with the column's coefficients (scale included) formed once
+                const T hs = (T)0.5 * sc;
+                const T wm = wxr[i] * wyr[j] * sc;
+                const T ax = dxd[i] * wyd[j] * hs, ay = wxd[i] * dyd[j] * hs, az = wxd[i] * wyd[j] * hs;
+                const T P0 = wm * f0, P1 = wm * f1, P2 = wm * f2;
+                const T Q0 = ay * t2, Q1 = -(ax * t2), Q2 = ax * t1 - ay * t0;
+                const T R0 = -(az * t1), R1 = az * t0;
                 for (int q = qa + phase; q < q1; q += P) {
                   const int qm = q + om, qd = q + od;
-                  const T zm = (qm >= 0 && qm < spm) ? zr[qm] : (T)0;
-                  T zA = (T)0, zB = (T)0;
-                  if (qd >= 0 && qd < spd) {
-                    zA = zAd[qd];
-                    zB = zBd[qd];
-                  }
-                  const T w = wm * zm;
-                  const T gx = ax * zA, gy = ay * zA, gz = az * zB;
+                  const T zm = zw(zr, qm, spm), zA = zw(zAd, qd, spd), zB = zw(zBd, qd, spd);
                   TA* d = dcol + q * PL;
-                  add(d, (w * f0 + h * (gy * t2 - gz * t1)));
-                  add(d + vol, (w * f1 + h * (gz * t0 - gx * t2)));
-                  add(d + 2 * vol, (w * f2 + h * (gx * t1 - gy * t0)));
+                  add(d, zm * P0 + zA * Q0 + zB * R0);
+                  add(d + vol, zm * P1 + zA * Q1 + zB * R1);
+                  add(d + 2 * vol, zm * P2 + zA * Q2);
                 }
               } else if (kind == 0) {
-                const T wxy = wxr[i] * wyr[j];
-                const T g0 = f0, g1 = f1, g2 = f2;
+                const T wxy = wxr[i] * wyr[j] * sc;
+                const T g0 = wxy * f0, g1 = wxy * f1, g2 = wxy * f2;
                 int q = qa + phase;
                 for (; q + 3 * P < q1; q += 4 * P) {
-                  const T z0 = zr[q], z1 = zr[q + P], z2 = zr[q + 2 * P], z3 = zr[q + 3 * P];
-                  const T w0 = wxy * z0, w1 = wxy * z1, w2 = wxy * z2, w3 = wxy * z3;
+                  const T w0 = zr[q], w1 = zr[q + P], w2 = zr[q + 2 * P], w3 = zr[q + 3 * P];
                   TA* d0 = dcol + q * PL;
                   add(d0, (w0 * g0)); add(d0 + vol, (w0 * g1));
                   add(d0 + 2 * vol, (w0 * g2));
@@ -691,39 +699,40 @@ __global__ void __launch_bounds__(256) k_spread(SpreadArgs A, const Tin* __restr
                   add(d3 + 2 * vol, (w3 * g2));
                 }
                 for (; q < q1; q += P) {
-                  const T w = wxy * zr[q];
+                  const T w = zr[q];
                   TA* d = dcol + q * PL;
                   add(d, (w * g0)); add(d + vol, (w * g1));
                   add(d + 2 * vol, (w * g2));
                 }
               } else {
-                const T h = (T)0.5;
-                const T ax = dxr[i] * wyr[j], ay = wxr[i] * dyr[j], az = wxr[i] * wyr[j];
+                // 1/2 g x f with g = (ax zA, ay zA, az zB): per component
+                // Q_c zA + R_c zB (R_2 = 0), coefficients formed once
+                const T hs = (T)0.5 * sc;
+                const T ax = dxr[i] * wyr[j] * hs, ay = wxr[i] * dyr[j] * hs, az = wxr[i] * wyr[j] * hs;
+                const T Q0 = ay * f2, Q1 = -(ax * f2), Q2 = ax * f1 - ay * f0;
+                const T R0 = -(az * f1), R1 = az * f0;
                 int q = qa + phase;
                 for (; q + P < q1; q += 2 * P) {
                   const T zA0 = zr[q], zB0 = zr[W + q], zA1 = zr[q + P], zB1 = zr[W + q + P];
                   {
-                    const T gx = ax * zA0, gy = ay * zA0, gz = az * zB0;
                     TA* d = dcol + q * PL;
-                    add(d, (h * (gy * f2 - gz * f1)));
-                    add(d + vol, (h * (gz * f0 - gx * f2)));
-                    add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
+                    add(d, zA0 * Q0 + zB0 * R0);
+                    add(d + vol, zA0 * Q1 + zB0 * R1);
+                    add(d + 2 * vol, zA0 * Q2);
                   }
                   {
-                    const T gx = ax * zA1, gy = ay * zA1, gz = az * zB1;
                     TA* d = dcol + (q + P) * PL;
-                    add(d, (h * (gy * f2 - gz * f1)));
-                    add(d + vol, (h * (gz * f0 - gx * f2)));
-                    add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
+                    add(d, zA1 * Q0 + zB1 * R0);
+                    add(d + vol, zA1 * Q1 + zB1 * R1);
+                    add(d + 2 * vol, zA1 * Q2);
                   }
                 }
                 for (; q < q1; q += P) {
                   const T zA = zr[q], zB = zr[W + q];
-                  const T gx = ax * zA, gy = ay * zA, gz = az * zB;
                   TA* d = dcol + q * PL;
-                  add(d, (h * (gy * f2 - gz * f1)));
-                  add(d + vol, (h * (gz * f0 - gx * f2)));
-                  add(d + 2 * vol, (h * (gx * f1 - gy * f0)));
+                  add(d, zA * Q0 + zB * R0);
+                  add(d + vol, zA * Q1 + zB * R1);
+                  add(d + 2 * vol, zA * Q2);
                 }
               }
             }
