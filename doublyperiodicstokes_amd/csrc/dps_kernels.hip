@@ -773,15 +773,85 @@ __device__ __forceinline__ T wave_sum(T v)
   return v;
 }
 
+// Column-stationary interpolation of one particle (supports of at most 64
+// columns): lane c owns support column c for the whole particle, its x/y
+// weight products formed once; the wavefront walks the rows together, so
+// each row's z weights are the same for every lane (scalar loads) and a row
+// costs three grid loads and the multiply-adds.  Lanes past the last
+// column add zeros.  Returns the lane's partial sums (wavefront-summed by
+// the caller): d[0..2] the monopole sums, d[3..11] the derivative sums
+// (x, y, z derivative kernels, three components each).
+//   zA: z values of the monopole rows sA.z + q (sA.w rows), zB / zBd: value
+//   and d/dz of the derivative rows sB.z + q; NB = 0 skips the derivative.
+template <class T, int NB>
+__device__ __forceinline__ void interp_columns(const InterpArgs& A, const T* __restrict__ U, int lane, int4 sA,
+                                               int4 sB, int wx, int wy, const T* awx, const T* awy,
+                                               const T* za, const T* bwx, const T* bwy, const T* bdx,
+                                               const T* bdy, const T* zb, const T* zbd, T (&d)[12])
+{
+#pragma unroll
+  for (int i = 0; i < 12; ++i) d[i] = 0;
+  const int ncol = wx * wy;
+  int lo = sA.z, hi = sA.z + sA.w;
+  if (NB && sB.w > 0) {
+    lo = (sA.w > 0) ? min(lo, sB.z) : sB.z;
+    hi = (sA.w > 0) ? max(hi, sB.z + sB.w) : sB.z + sB.w;
+  }
+  const int nz = hi - lo, oa = lo - sA.z, ob = lo - sB.z;
+  if (lane >= ncol) return;
+  const int b = lane / wx, a = lane - b * wx;
+  int x = sA.x + a, y = sA.y + b;
+  x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+  y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+  const T wa = awx[a] * awy[b];
+  T gxw = 0, gyw = 0, gzw = 0;
+  if (NB) {
+    gxw = bdx[a] * bwy[b];
+    gyw = bwx[a] * bdy[b];
+    gzw = bwx[a] * bwy[b];
+  }
+  const unsigned zs = (unsigned)A.Ny * (unsigned)A.Nx;
+  const size_t cs = (size_t)A.Nz * zs;
+  const T* U1 = U + cs;
+  const T* U2 = U + 2 * cs;
+  const unsigned col = (unsigned)y * (unsigned)A.Nx + (unsigned)x;
+#pragma unroll 2
+  for (int q = 0; q < nz; ++q) {
+    int zr_ = lo + q;
+    if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+    const unsigned o = (unsigned)zr_ * zs + col;
+    const T v0 = U[o], v1 = U1[o], v2 = U2[o];
+    const int qa = q + oa;
+    const T w = ((unsigned)qa < (unsigned)sA.w) ? wa * za[qa] : (T)0;
+    d[0] += w * v0; d[1] += w * v1; d[2] += w * v2;
+    if (NB) {
+      const int qb = q + ob;
+      T zv = 0, zd = 0;
+      if ((unsigned)qb < (unsigned)sB.w) {
+        zv = zb[qb];
+        zd = zbd[qb];
+      }
+      const T gx = gxw * zv, gy = gyw * zv, gz = gzw * zd;
+      d[3] += gx * v0; d[4] += gx * v1; d[5] += gx * v2;
+      d[6] += gy * v0; d[7] += gy * v1; d[8] += gy * v2;
+      d[9] += gz * v0; d[10] += gz * v1; d[11] += gz * v2;
+    }
+  }
+}
+
 // Interpolation: one wavefront per particle (tile order, mapped so that each
-// XCD handles a contiguous run of tiles), one lane per (column, z) point of
-// the folded support, wavefront sums.  Ω from the derivative kernels:
+// XCD handles a contiguous run of tiles), wavefront sums.  Supports of at
+// most 64 columns take the column-stationary form above; wider ones one
+// lane per (column, z) point of the folded support.  Ω from the derivative
+// kernels:
 // omega = -1/2 (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)  (FCM.py:424-426)
 template <class T, class Tout>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__ vel,
                                                 Tout* __restrict__ ang)
 {
-  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // the particle index is the same for the whole wavefront: scalar registers
+  const int g = __builtin_amdgcn_readfirstlane(xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                                               (int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (g >= A.N) return;
   const int p = A.order[g];
@@ -803,6 +873,21 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     const T* ddy = ddx + wx;
     const T* dzi = ddy + wy + 2 * A.kd.wzmax;   // value, then d/dz, rows sd.z + q
     const T* ddzi = dzi + A.kd.wzmax;
+    if (wx * wy <= 64) {
+      T d[12];
+      interp_columns<T, 1>(A, U, lane, sm, sd, wx, wy, mwx, mwy, mzi, dwx, dwy, ddx, ddy, dzi, ddzi, d);
+#pragma unroll
+      for (int i = 0; i < 12; ++i) d[i] = wave_sum(d[i]);
+      if (lane == 0) {
+        vel[3 * p] = (Tout)d[0];
+        vel[3 * p + 1] = (Tout)d[1];
+        vel[3 * p + 2] = (Tout)d[2];
+        ang[3 * p] = (Tout)((T)(-0.5) * (d[8] - d[10]));
+        ang[3 * p + 1] = (Tout)((T)(-0.5) * (d[9] - d[5]));
+        ang[3 * p + 2] = (Tout)((T)(-0.5) * (d[4] - d[6]));
+      }
+      return;
+    }
     int lo = sm.z, hi = sm.z + sm.w;
     if (sd.w > 0) {
       lo = (sm.w > 0) ? min(lo, sd.z) : sd.z;
@@ -867,6 +952,32 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
     const T* wzi = zr + (kind ? 2 : 1) * W;   // interpolation weights (value kernel)
     const T* dzi = zr + 3 * W;                // d/dz kernel (dipole only)
     const int ncol = wx * wy;
+    if (ncol <= 64) {
+      T e[12];
+      if (kind == 0) {
+        interp_columns<T, 0>(A, U, lane, sup, sup, wx, wy, wxr, wyr, wzi, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, e);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] = wave_sum(e[i]);
+        if (lane == 0) {
+          vel[3 * p] = (Tout)e[0];
+          vel[3 * p + 1] = (Tout)e[1];
+          vel[3 * p + 2] = (Tout)e[2];
+        }
+      } else {
+        // derivative kernels only: no monopole rows
+        const int4 s0 = make_int4(sup.x, sup.y, sup.z, 0);
+        interp_columns<T, 1>(A, U, lane, s0, sup, wx, wy, wxr, wyr, wzi, wxr, wyr, dxr, dyr, wzi, dzi, e);
+#pragma unroll
+        for (int i = 3; i < 12; ++i) e[i] = wave_sum(e[i]);
+        if (lane == 0) {
+          ang[3 * p] = (Tout)((T)(-0.5) * (e[8] - e[10]));
+          ang[3 * p + 1] = (Tout)((T)(-0.5) * (e[9] - e[5]));
+          ang[3 * p + 2] = (Tout)((T)(-0.5) * (e[4] - e[6]));
+        }
+      }
+      continue;
+    }
     const int npts = ncol * nz;
     const float incol = 1.0f / (float)ncol, iwx = 1.0f / (float)wx;
     T d[9];
