@@ -23,8 +23,18 @@ template <class T, int N> struct FftTile {
   typedef typename C2<T>::t V;
   // 256 bytes of each row per access; the LDS tile (N x TW complex) within 64 KB
   static constexpr int tw0 = 256 / (int)sizeof(V);
-  static constexpr int value = (N * tw0 * (int)sizeof(V) <= 65536) ? tw0
-                               : (N * (tw0 / 2) * (int)sizeof(V) <= 65536) ? tw0 / 2 : tw0 / 4;
+  static constexpr int value = (N * tw0 * (int)sizeof(V) <= 65536)         ? tw0
+                               : (N * (tw0 / 2) * (int)sizeof(V) <= 65536) ? tw0 / 2
+                               : (N * (tw0 / 4) * (int)sizeof(V) <= 65536) ? tw0 / 4
+                                                                           : tw0 / 8;
+};
+
+// threads per workgroup: one column group per row of the longer pass, at
+// most 512 (more loads in flight per CU)
+template <class T, int R1, int R2> struct FftThreads {
+  static constexpr int TW = FftTile<T, R1 * R2>::value;
+  static constexpr int R = R1 > R2 ? R1 : R2;
+  static constexpr int value = TW * R < 512 ? TW * R : 512;
 };
 
 template <class T, int R1, int R2, int DIR>
@@ -34,7 +44,7 @@ __global__ void __launch_bounds__(512) k_fft_y(typename C2<T>::t* __restrict__ d
   typedef typename C2<T>::t V;
   constexpr int N = R1 * R2;
   constexpr int TW = FftTile<T, N>::value;
-  constexpr int G1 = 512 / TW;   // column groups (512 threads: more loads in flight per CU)
+  constexpr int G1 = FftThreads<T, R1, R2>::value / TW;   // column groups
   __shared__ V buf[N * TW];       // [n1][k2][col]
   const int col = threadIdx.x % TW, g = threadIdx.x / TW;
   // neighbouring tiles share the cache lines at their row ends (rows of Nkx
@@ -97,19 +107,20 @@ void fft_y_launch(void* data, int nslices, int Nkx, const void* tw, int dir, hip
 {
   typedef typename C2<T>::t V;
   constexpr int TW = FftTile<T, R1 * R2>::value;
+  constexpr int NT = FftThreads<T, R1, R2>::value;
   const int ncols = nslices * Nkx;
   const dim3 grid((ncols + TW - 1) / TW);
   if (dir < 0)
-    hipLaunchKernelGGL((k_fft_y<T, R1, R2, -1>), grid, dim3(512), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
+    hipLaunchKernelGGL((k_fft_y<T, R1, R2, -1>), grid, dim3(NT), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
   else
-    hipLaunchKernelGGL((k_fft_y<T, R1, R2, 1>), grid, dim3(512), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
+    hipLaunchKernelGGL((k_fft_y<T, R1, R2, 1>), grid, dim3(NT), 0, s, (V*)data, ncols, Nkx, (const V*)tw);
 }
 
 }  // namespace
 
 bool fft_y_supported(int Ny)
 {
-  return Ny == 16 || Ny == 32 || Ny == 64 || Ny == 128 || Ny == 256 || Ny == 512;
+  return Ny == 16 || Ny == 32 || Ny == 64 || Ny == 128 || Ny == 256 || Ny == 512 || Ny == 1024;
 }
 
 template <class T>
@@ -122,6 +133,7 @@ void launch_fft_y(void* data, int nslices, int Nkx, int Ny, const void* tw, int 
     case 128: fft_y_launch<T, 16, 8>(data, nslices, Nkx, tw, dir, s); break;
     case 256: fft_y_launch<T, 16, 16>(data, nslices, Nkx, tw, dir, s); break;
     case 512: fft_y_launch<T, 32, 16>(data, nslices, Nkx, tw, dir, s); break;
+    case 1024: fft_y_launch<T, 32, 32>(data, nslices, Nkx, tw, dir, s); break;
     default: break;   // callers check fft_y_supported
   }
 }

@@ -16,6 +16,7 @@ CASES = [
     ("fy128", "bottom", 12.0, 64.0, 6.0, 24, 128, 11, 30, True),
     ("fy256_oddx", "slit", 7.5, 128.0, 5.0, 15, 256, 9, 30, True),
     ("fy512", "nowall", 8.0, 256.0, 6.0, 16, 512, 11, 30, True),
+    ("fy1024_oddx", "bottom", 6.5, 512.0, 6.0, 13, 1024, 9, 30, True),
 ]
 
 
