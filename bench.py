@@ -23,6 +23,9 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# MI355X FP32 peak (dense matrix cores; the vector ALUs reach it only with
+# packed FP32, which this build does not emit: half of it without)
+FP32_PEAK_TFLOPS = 157.3
 HERE = os.path.dirname(os.path.abspath(__file__))
 # kernel that runs each stage (the "bvp" stage is the fused spectral kernel)
 STAGE_KERNEL = {"spread": "k_spread", "fft_fwd": "rocfft x + k_fft_y", "z_fwd": "k_spectral",
@@ -78,6 +81,17 @@ def stage_bytes(wl, s, reg=False):
         "fft_inv": g * (spec + ng * s),
         "interp": g * ng * s + npart * s,
     }
+
+
+def z_transform_flop(wl):
+    """FLOPs of the forward and inverse Chebyshev z transforms of one Mdot:
+    per mode, component and real part, Nz coefficients each a sum over the
+    ceil(Nz/2) folded node pairs (forward), and Nz node values from the
+    ceil(Nz/2) even/odd coefficient pairs (inverse); one FMA = 2 FLOPs."""
+    geom, N, Lx, Ly, zmax, zlo, zhi, nx, ny, nz, torque = wl
+    nm = ny * (nx // 2 + 1)
+    nh = (nz + 1) // 2
+    return 2 * (2 * nz * nh * 2 * 3 * nm)
 
 
 def gpu_solver(wl, precision, reg=False, solver="auto"):
@@ -388,6 +402,13 @@ def main():
                      "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": sb[dom],
                      "launch_us": stage_us[dom]},
     }
+    if kernels["bvp"] == "k_spectral" and not reg:
+        # the Chebyshev z transforms inside the fused solve: FP32 VALU sums
+        # over folded node pairs, forward and inverse
+        zf = z_transform_flop(wl)
+        out["z_transform"] = {"kernel": "k_spectral", "unit": "VALU FP32", "flop_per_launch": zf,
+                              "tflops_over_kernel_time": zf / (stage_us["bvp"] * 1e-6) / 1e12,
+                              "peak_tflops": FP32_PEAK_TFLOPS}
     del mixed
     U64 = W64 = None
     if not args.no_fp64:
