@@ -167,6 +167,8 @@ struct dps_handle {
 
   rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
   bool fft_y = false;        // rocFFT runs the x pass only; k_fft_y the y pass
+  bool fft_x = false;        // with fft_y: k_fft_x runs the x pass (no rocFFT plan)
+  void* d_twx = nullptr;     // x-pass twiddles exp(-2 pi i k / Nx)
   void* d_twy = nullptr;     // y-pass twiddles exp(-2 pi i k / Ny)
   rocfft_execution_info info = nullptr;
   void* fft_work = nullptr;
@@ -695,6 +697,18 @@ int make_plans(dps_handle* h)
     else rc = h->upload((double**)&h->d_twy, tw);
     if (rc) return rc;
   }
+  // power-of-two Nx as well: k_fft_x does the x pass and no rocFFT plan is made
+  h->fft_x = h->fft_y && fft_x_supported(h->Nx);
+  if (h->fft_x) {
+    std::vector<double> tw(2 * (size_t)h->Nx);
+    for (int k = 0; k < h->Nx; ++k) {
+      tw[2 * k] = std::cos(2 * M_PI * k / h->Nx);
+      tw[2 * k + 1] = -std::sin(2 * M_PI * k / h->Nx);
+    }
+    if (h->prec == DPS_PREC_MIXED) rc = h->upload((float**)&h->d_twx, cast_vec<float>(tw));
+    else rc = h->upload((double**)&h->d_twx, tw);
+    return rc;
+  }
   FFTCHK(rocfft_plan_create(&h->plan_fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
                             fp, dim, lengths, batch, nullptr));
   FFTCHK(rocfft_plan_create(&h->plan_inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
@@ -722,10 +736,17 @@ int run_fft(dps_handle* h, bool forward, void* in, void* out, hipStream_t s)
     else launch_fft_y<double>(in, ns, h->Nkx, h->Ny, h->d_twy, 1, s);
     HIPCHK(hipGetLastError());
   }
-  FFTCHK(rocfft_execution_info_set_stream(h->info, s));
-  void* ib[1] = {in};
-  void* ob[1] = {out};
-  FFTCHK(rocfft_execute(forward ? h->plan_fwd : h->plan_inv, ib, ob, h->info));
+  if (h->fft_x) {
+    const int rows = ns * h->Ny;
+    if (mixed(h)) launch_fft_x<float>(in, out, rows, h->Nx, h->d_twx, forward ? -1 : 1, s);
+    else launch_fft_x<double>(in, out, rows, h->Nx, h->d_twx, forward ? -1 : 1, s);
+    HIPCHK(hipGetLastError());
+  } else {
+    FFTCHK(rocfft_execution_info_set_stream(h->info, s));
+    void* ib[1] = {in};
+    void* ob[1] = {out};
+    FFTCHK(rocfft_execute(forward ? h->plan_fwd : h->plan_inv, ib, ob, h->info));
+  }
   if (h->fft_y && forward) {
     if (mixed(h)) launch_fft_y<float>(out, ns, h->Nkx, h->Ny, h->d_twy, -1, s);
     else launch_fft_y<double>(out, ns, h->Nkx, h->Ny, h->d_twy, -1, s);

@@ -144,6 +144,14 @@ void launch_tp_curl(const void* in, void* out, int Nx, int Ny, int Nz, double Lx
 // y pass of the planar transforms: in-place c2c of length Ny along ky of the
 // spectrum [nslices][Ny][Nkx], dir -1 forward / +1 inverse (unnormalised);
 // tw holds exp(-2 pi i k / Ny), k < Ny, in grid precision.
+// x pass of the planar transforms: real rows of NX = Nx points ([row][x]) to
+// half-complex rows of Nx/2 + 1 values ([row][kx]), dir -1 forward (real ->
+// complex), +1 inverse (complex -> real, unnormalised, imaginary parts of
+// kx = 0 and kx = Nx/2 ignored); tw holds exp(-2 pi i k / Nx), k < Nx, in
+// grid precision.  Out of place.
+bool fft_x_supported(int Nx);
+template <class T>
+void launch_fft_x(const void* in, void* out, int nrows, int Nx, const void* tw, int dir, hipStream_t s);
 bool fft_y_supported(int Ny);
 template <class T>
 void launch_fft_y(void* data, int nslices, int Nkx, int Ny, const void* tw, int dir, hipStream_t s);

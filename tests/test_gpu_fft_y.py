@@ -1,6 +1,7 @@
 """The planar transforms with the y pass in k_fft_y (every power-of-two Ny the
 kernel handles, odd and even Nx so that the flattened (slice, kx) column tiles
-straddle slices at different offsets) against the FP64 oracle, through Mdot
+straddle slices at different offsets) and, for power-of-two Nx, the x pass in
+k_fft_x (every length it handles), against the FP64 oracle, through Mdot
 with torques.  Bounds as everywhere: rel-L2 <= 1e-11 FP64, <= 1e-5 mixed."""
 import numpy as np
 import pytest
@@ -17,6 +18,11 @@ CASES = [
     ("fy256_oddx", "slit", 7.5, 128.0, 5.0, 15, 256, 9, 30, True),
     ("fy512", "nowall", 8.0, 256.0, 6.0, 16, 512, 11, 30, True),
     ("fy1024_oddx", "bottom", 6.5, 512.0, 6.0, 13, 1024, 9, 30, True),
+    # power-of-two Nx as well: the x pass in k_fft_x (Nx/2 = 4x4 ... 32x32)
+    ("fx32", "slit", 16.0, 16.0, 6.3, 32, 32, 13, 20, True),
+    ("fx64", "slit", 32.0, 16.0, 6.3, 64, 32, 13, 30, True),
+    ("fx512", "bottom", 256.0, 8.0, 6.0, 512, 16, 11, 30, True),
+    ("fx2048", "nowall", 1024.0, 8.0, 6.0, 2048, 16, 9, 30, True),
 ]
 
 
