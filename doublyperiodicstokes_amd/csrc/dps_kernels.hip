@@ -1028,10 +1028,12 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs A, Tout* __restrict__
 }
 
 // Position gradient of the adjoint (no counterpart in the reference): one
-// wavefront per particle, a lane per support point, the derivative records of
-// the force kernel in the dipole layout ([wx][wy][dx][dy][z][dz][z interp]
-// [dz interp]); both grids are read at each point and the two 3x3 gradient
-// tensors reduced over the wavefront.
+// wavefront per particle, the derivative records of the force kernel in the
+// dipole layout ([wx][wy][dx][dy][z][dz][z interp][dz interp]); both grids
+// are read at each point and the two 3x3 gradient tensors reduced over the
+// wavefront.  Supports of at most 64 columns are column-stationary (a lane
+// per column, its x/y products formed once, the rows walked together with
+// wave-uniform z weights); wider ones take a lane per support point.
 template <class T, class Tio>
 __global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* __restrict__ sup_g,
                                                      const T* __restrict__ recs, int rs, KernelDesc K,
@@ -1039,7 +1041,8 @@ __global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* _
                                                      const Tio* __restrict__ lam, const Tio* __restrict__ f,
                                                      Tio* __restrict__ grad)
 {
-  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int g = __builtin_amdgcn_readfirstlane(xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                                               (int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (g >= A.N) return;
   const int p = A.order[g];
@@ -1061,6 +1064,35 @@ __global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* _
   T du[9], dv[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) du[i] = dv[i] = 0;
+  if (ncol <= 64) {
+    if (lane < ncol) {
+      const int b = lane / wx, a = lane - b * wx;
+      int x = sup.x + a, y = sup.y + b;
+      x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+      y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+      const T cx = dxr[a] * wyr[b], cy = wxr[a] * dyr[b], cz = wxr[a] * wyr[b];
+      const unsigned zsu = (unsigned)zs, col = (unsigned)y * (unsigned)A.Nx + (unsigned)x;
+      const T* U1 = U + cs;
+      const T* U2 = U + 2 * cs;
+      const T* V1 = V + cs;
+      const T* V2 = V + 2 * cs;
+#pragma unroll 2
+      for (int q = 0; q < nz; ++q) {
+        int zr_ = sup.z + q;
+        if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+        const unsigned o = (unsigned)zr_ * zsu + col;
+        const T u0 = U[o], u1 = U1[o], u2 = U2[o];
+        const T v0 = V[o], v1 = V1[o], v2 = V2[o];
+        const T gx = cx * wzi[q], gy = cy * wzi[q], gz = cz * dzi[q];
+        du[0] += gx * u0; du[1] += gx * u1; du[2] += gx * u2;
+        du[3] += gy * u0; du[4] += gy * u1; du[5] += gy * u2;
+        du[6] += gz * u0; du[7] += gz * u1; du[8] += gz * u2;
+        dv[0] += gx * v0; dv[1] += gx * v1; dv[2] += gx * v2;
+        dv[3] += gy * v0; dv[4] += gy * v1; dv[5] += gy * v2;
+        dv[6] += gz * v0; dv[7] += gz * v1; dv[8] += gz * v2;
+      }
+    }
+  } else
   for (int t = lane; t < npts; t += 64) {
     int col, a;
     const int q = fdiv(t, ncol, incol, &col);
@@ -1196,7 +1228,8 @@ __global__ void __launch_bounds__(256) k_interp_grad_t(InterpArgs A, const int4*
                                                        const Tio* __restrict__ mu, const Tio* __restrict__ tq,
                                                        Tio* __restrict__ grad)
 {
-  const int g = xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int g = __builtin_amdgcn_readfirstlane(xcd_group(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                                               (int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (g >= A.N) return;
   const int p = A.order[g];
@@ -1224,6 +1257,40 @@ __global__ void __launch_bounds__(256) k_interp_grad_t(InterpArgs A, const int4*
   T au[27], av[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i) au[i] = av[i] = 0;
+  if (ncol <= 64) {
+    // column-stationary, as in k_interp_grad: the nine x/y products of J
+    // formed once per lane, the z factor per row uniform over the wavefront
+    if (lane < ncol) {
+      const int b = lane / wx, a = lane - b * wx;
+      int x = sup.x + a, y = sup.y + b;
+      x += (x < 0) ? A.Nx : ((x >= A.Nx) ? -A.Nx : 0);
+      y += (y < 0) ? A.Ny : ((y >= A.Ny) ? -A.Ny : 0);
+      T C[9];
+      C[0] = xdd[a] * yw[b]; C[1] = xp[a] * yf[b]; C[2] = xp[a] * yw[b];
+      C[3] = xf[a] * yp[b]; C[4] = xw[a] * ydd[b]; C[5] = xw[a] * yp[b];
+      C[6] = xf[a] * yw[b]; C[7] = xw[a] * yf[b]; C[8] = xw[a] * yw[b];
+      const unsigned zsu = (unsigned)zs, col = (unsigned)y * (unsigned)A.Nx + (unsigned)x;
+      for (int q = 0; q < nz; ++q) {
+        int zr_ = sup.z + q;
+        if (A.periodic_z) zr_ += (zr_ < 0) ? A.Nz : ((zr_ >= A.Nz) ? -A.Nz : 0);
+        const unsigned o = (unsigned)zr_ * zsu + col;
+        const T u[3] = {U[o], U[o + cs], U[o + 2 * cs]};
+        const T v[3] = {V[o], V[o + cs], V[o + 2 * cs]};
+        const T za = zA[q], zpa = zpA[q], zfb = zfB[q], zddb = zddB[q];
+        T J[9];
+        J[0] = C[0] * za; J[1] = C[1] * za; J[2] = C[2] * zfb;
+        J[3] = C[3] * za; J[4] = C[4] * za; J[5] = C[5] * zfb;
+        J[6] = C[6] * zpa; J[7] = C[7] * zpa; J[8] = C[8] * zddb;
+#pragma unroll
+        for (int kb = 0; kb < 9; ++kb)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            au[kb * 3 + c] += J[kb] * u[c];
+            av[kb * 3 + c] += J[kb] * v[c];
+          }
+      }
+    }
+  } else
   for (int t = lane; t < npts; t += 64) {
     int col, a;
     const int q = fdiv(t, ncol, incol, &col);
