@@ -1140,11 +1140,23 @@ __global__ void __launch_bounds__(256) k_interp_grad(InterpArgs A, const int4* _
 // class A][z dpos, A][z d, class B][z dd, B] (z with the interpolation
 // weights hx hy w_z, on the interior rows sup.z .. sup.z + sup.w - 1 of the
 // torque kernel's own records).
+//
+// Four particles per wavefront, 16 lanes each, as k_stencil: lanes form the x
+// and y points, and the entries of the extended z support (kept in LDS with
+// the interior row each folds onto); lane r then adds row r's contributions
+// in entry order, in record precision, as a serial pass adding them to a
+// zeroed record would.  nq bounds the entries of the z support.
 template <class T>
-__global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* __restrict__ rec_out, int rs)
+__global__ void __launch_bounds__(64) k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d,
+                                                   T* __restrict__ rec_out, int rs, int nq)
 {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.N) return;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int sl = threadIdx.x & 15, pg = threadIdx.x >> 4;
+  double* zv = reinterpret_cast<double*>(smem_raw) + pg * 4 * nq;    // [4][nq] contributions of each entry
+  int* zrow = reinterpret_cast<int*>(reinterpret_cast<double*>(smem_raw) + 16 * nq) + pg * nq;   // [nq] row, -1: none
+  const int p0 = blockIdx.x * 4 + pg;
+  const bool valid = p0 < A.N;   // the last block's spare groups read particle N-1 and write nothing
+  const int p = valid ? p0 : A.N - 1;
   const KernelDesc& K = A.kd;
   const double x = A.pos[3 * p], y = A.pos[3 * p + 1], z = A.pos[3 * p + 2];
   const int4 sup = sup_d[p];
@@ -1161,61 +1173,80 @@ __global__ void k_stencil_d2(StencilArgs A, const int4* __restrict__ sup_d, T* _
     *dp = cl ? 0.0 : ph * d;
     *dd = cl ? 0.0 : ph * (d * d - bw / (a2 * s * s * s));
   };
-  int o = 0;
-  {
-    const int wx = K.wx;
-    for (int i = 0; i < wx; ++i) {
+  const int wx = K.wx, wy = K.wy;
+  if (valid) {
+    for (int i = sl; i < wx; i += 16) {
       double w, df, dp, dd;
       axis(fma((double)(sup.x + i), A.hx, -x), K.inv_norm3, &w, &df, &dp, &dd);
-      rec[o + i] = (T)w;
-      rec[o + wx + i] = (T)df;
-      rec[o + 2 * wx + i] = (T)dp;
-      rec[o + 3 * wx + i] = (T)dd;
+      rec[i] = (T)w;
+      rec[wx + i] = (T)df;
+      rec[2 * wx + i] = (T)dp;
+      rec[3 * wx + i] = (T)dd;
     }
-    o += 4 * wx;
-    const int wy = K.wy;
-    for (int j = 0; j < wy; ++j) {
+    for (int j = sl; j < wy; j += 16) {
       double w, df, dp, dd;
       axis(fma((double)(sup.y + j), A.hy, -y), 1.0, &w, &df, &dp, &dd);
-      rec[o + j] = (T)w;
-      rec[o + wy + j] = (T)df;
-      rec[o + 2 * wy + j] = (T)dp;
-      rec[o + 3 * wy + j] = (T)dd;
+      rec[4 * wx + j] = (T)w;
+      rec[4 * wx + wy + j] = (T)df;
+      rec[4 * wx + 2 * wy + j] = (T)dp;
+      rec[4 * wx + 3 * wy + j] = (T)dd;
     }
-    o += 4 * wy;
   }
+  const int o = 4 * (wx + wy);
   const int W = K.wzmax;
-  for (int i = 0; i < 4 * W; ++i) rec[o + i] = (T)0;
   const double hxhy = A.hx * A.hy;
   if (A.periodic_z) {
-    for (int q = 0; q < sup.w; ++q) {
-      double w, df, dp, dd;
-      axis(fma((double)(sup.z + q), A.hz, -z), 1.0, &w, &df, &dp, &dd);
-      const double qw = hxhy * A.hz;
-      rec[o + q] = (T)(w * qw);
-      rec[o + W + q] = (T)(dp * qw);
-      rec[o + 2 * W + q] = (T)(df * qw);
-      rec[o + 3 * W + q] = (T)(dd * qw);
-    }
+    if (valid)
+      for (int q = sl; q < W; q += 16) {
+        double w = 0, df = 0, dp = 0, dd = 0;
+        if (q < sup.w) axis(fma((double)(sup.z + q), A.hz, -z), 1.0, &w, &df, &dp, &dd);
+        const double qw = hxhy * A.hz;
+        rec[o + q] = (T)(w * qw);
+        rec[o + W + q] = (T)(dp * qw);
+        rec[o + 2 * W + q] = (T)(df * qw);
+        rec[o + 3 * W + q] = (T)(dd * qw);
+      }
     return;
   }
   // doubly periodic: extended nodes folded onto the interior rows with the
   // value kernel's factors (class A) or the z-derivative's (class B)
-  const Stencil st = stencil_support(x, y, z, A.hx, A.hy, K, A.zext_d);
-  for (int q = 0; q < st.nz; ++q) {
+  Stencil st = grp_stencil_support(x, y, z, A.hx, A.hy, K, A.zext_d, sl);
+  if (st.nz > nq) st.nz = nq;   // cannot happen (nq bounds the window)
+  for (int q = sl; q < st.nz; q += 16) {
     const int e = st.e0 + q;
     const double sa = A.fA_d[e], sb = A.fB_d[e];
-    if (sa == 0.0 && sb == 0.0) continue;
-    const int r = A.kt_d[e] - sup.z;
-    if (r < 0 || r >= sup.w) continue;   // cannot happen: sup spans every row with a factor
-    double w, df, dp, dd;
-    axis(A.zext_d[e] - z, 1.0, &w, &df, &dp, &dd);
-    const double qw = hxhy * A.zw_d[e];
-    rec[o + r] += (T)(w * qw * sa);
-    rec[o + W + r] += (T)(dp * qw * sa);
-    rec[o + 2 * W + r] += (T)(df * qw * sb);
-    rec[o + 3 * W + r] += (T)(dd * qw * sb);
+    int r = -1;
+    if (!(sa == 0.0 && sb == 0.0)) {
+      r = A.kt_d[e] - sup.z;
+      if (r < 0 || r >= sup.w) r = -1;   // cannot happen: sup spans every row with a factor
+    }
+    if (r >= 0) {
+      double w, df, dp, dd;
+      axis(A.zext_d[e] - z, 1.0, &w, &df, &dp, &dd);
+      const double qw = hxhy * A.zw_d[e];
+      zv[q] = w * qw * sa;
+      zv[nq + q] = dp * qw * sa;
+      zv[2 * nq + q] = df * qw * sb;
+      zv[3 * nq + q] = dd * qw * sb;
+    }
+    zrow[q] = r;
   }
+  __syncthreads();
+  if (valid)
+    for (int r = sl; r < W; r += 16) {
+      T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0;
+      for (int q = 0; q < st.nz; ++q) {
+        if (zrow[q] != r) continue;
+        c0 += (T)zv[q];
+        c1 += (T)zv[nq + q];
+        c2 += (T)zv[2 * nq + q];
+        c3 += (T)zv[3 * nq + q];
+      }
+      rec[o + r] = c0;
+      rec[o + W + r] = c1;
+      rec[o + 2 * W + r] = c2;
+      rec[o + 3 * W + r] = c3;
+    }
 }
 
 // Torque terms of the position gradient: with J_kb the interpolation with the
@@ -2057,7 +2088,10 @@ template <class T>
 void launch_stencil_d2(const StencilArgs& A, const int4* sup_d, void* rec, int rs, hipStream_t s)
 {
   if (A.N == 0) return;
-  hipLaunchKernelGGL((k_stencil_d2<T>), dim3((A.N + 127) / 128), dim3(128), 0, s, A, sup_d, (T*)rec, rs);
+  // four particles per 64-lane block, each with [4][nq] doubles and [nq] ints of LDS
+  const int nq = std::max(1, A.kd.wzmax);
+  hipLaunchKernelGGL((k_stencil_d2<T>), dim3((A.N + 3) / 4), dim3(64),
+                     (size_t)4 * nq * (4 * sizeof(double) + sizeof(int)), s, A, sup_d, (T*)rec, rs, nq);
 }
 
 template <class T, class Tio>
