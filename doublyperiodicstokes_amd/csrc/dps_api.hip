@@ -682,7 +682,7 @@ int make_plans(dps_handle* h)
   const rocfft_precision fp = (h->prec == DPS_PREC_MIXED) ? rocfft_precision_single : rocfft_precision_double;
   const bool tp = h->geometry == DPS_MODE_PERIODIC;
   size_t lengths[3] = {(size_t)h->Nx, (size_t)h->Ny, (size_t)h->Nz};
-  // doubly periodic with a power-of-two Ny: rocFFT does the unit-stride x pass
+  // doubly periodic with Ny a power of two or 3 * 2^k: rocFFT does the unit-stride x pass
   // (batched over component, z and y) and k_fft_y the strided y pass
   h->fft_y = !tp && fft_y_supported(h->Ny);
   const size_t dim = tp ? 3 : (h->fft_y ? 1 : 2);
@@ -697,7 +697,7 @@ int make_plans(dps_handle* h)
     else rc = h->upload((double**)&h->d_twy, tw);
     if (rc) return rc;
   }
-  // power-of-two Nx as well: k_fft_x does the x pass and no rocFFT plan is made
+  // Nx a power of two or 3 * 2^k as well: k_fft_x does the x pass and no rocFFT plan is made
   h->fft_x = h->fft_y && fft_x_supported(h->Nx);
   if (h->fft_x) {
     std::vector<double> tw(2 * (size_t)h->Nx);

@@ -36,13 +36,86 @@ constexpr int fft_brev(int i, int R)
   return r;
 }
 
-// In-place DFT of R points (R = 2, 4, 8, 16 or 32) held in registers:
-// X_k = sum_n x_n exp(DIR 2 pi i n k / R), DIR = -1 forward, +1 inverse
-// (unnormalised).  Radix-2 decimation in time; every twiddle is a compile-time
-// constant once the loops are unrolled, and the trivial ones are skipped.
+// cos(2 pi j / 24), j = 0..23
+constexpr double kCos24[24] = {
+    1.0,
+    0.96592582628906828674974319972890,
+    0.86602540378443864676372317075294,
+    0.70710678118654752440084436210485,
+    0.5,
+    0.25881904510252076234889883762405,
+    0.0,
+    -0.25881904510252076234889883762405,
+    -0.5,
+    -0.70710678118654752440084436210485,
+    -0.86602540378443864676372317075294,
+    -0.96592582628906828674974319972890,
+    -1.0,
+    -0.96592582628906828674974319972890,
+    -0.86602540378443864676372317075294,
+    -0.70710678118654752440084436210485,
+    -0.5,
+    -0.25881904510252076234889883762405,
+    0.0,
+    0.25881904510252076234889883762405,
+    0.5,
+    0.70710678118654752440084436210485,
+    0.86602540378443864676372317075294,
+    0.96592582628906828674974319972890};
+
+template <int R, int DIR, class T> DPS_HD void fft_reg(T (&re)[R], T (&im)[R]);
+
+// DFT of R = 3 P points (P = 1, 2, 4 or 8) held in registers: the three
+// P-point DFTs of the decimated sequences x[3 n2 + n1], then
+//   X_k = sum_{n1 < 3} exp(DIR 2 pi i n1 k / R) A_{n1}[k mod P]
+// with the twiddles cos/sin(2 pi m / 24), m = (24 / R) n1 k mod 24,
+// compile-time constants once the loops are unrolled.
+template <int R, int DIR, class T>
+DPS_HD void dft_reg3(T (&re)[R], T (&im)[R])
+{
+  static_assert(R == 3 || R == 6 || R == 12 || R == 24, "R must be 3 times 1, 2, 4 or 8");
+  constexpr int P = R / 3;
+  T ar[3][P], ai[3][P];
+#pragma unroll
+  for (int n1 = 0; n1 < 3; ++n1)
+#pragma unroll
+    for (int n2 = 0; n2 < P; ++n2) {
+      ar[n1][n2] = re[3 * n2 + n1];
+      ai[n1][n2] = im[3 * n2 + n1];
+    }
+  if constexpr (P > 1) {
+#pragma unroll
+    for (int n1 = 0; n1 < 3; ++n1) fft_reg<P, DIR>(ar[n1], ai[n1]);
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    T xr = ar[0][k % P], xi = ai[0][k % P];
+#pragma unroll
+    for (int n1 = 1; n1 < 3; ++n1) {
+      const int m = ((24 / R) * n1 * k) % 24;
+      const T c = (T)kCos24[m];
+      const T sn = (T)(DIR * kCos24[(m + 18) % 24]);   // sin(2 pi m / 24) = cos(2 pi (m - 6) / 24)
+      const T yr = ar[n1][k % P], yi = ai[n1][k % P];
+      xr += yr * c - yi * sn;
+      xi += yr * sn + yi * c;
+    }
+    re[k] = xr;
+    im[k] = xi;
+  }
+}
+
+// In-place DFT of R points held in registers: X_k = sum_n x_n exp(DIR 2 pi
+// i n k / R), DIR = -1 forward, +1 inverse (unnormalised).  R = 2, 4, 8, 16
+// or 32: radix-2 decimation in time; every twiddle is a compile-time constant
+// once the loops are unrolled, and the trivial ones are skipped.  R = 3, 6,
+// 12 or 24: dft_reg3 (three R/3-point transforms and a radix-3 combination).
 template <int R, int DIR, class T>
 DPS_HD void fft_reg(T (&re)[R], T (&im)[R])
 {
+  if constexpr (R % 3 == 0) {
+    dft_reg3<R, DIR>(re, im);
+    return;
+  } else {
   static_assert(R >= 2 && R <= 32 && (R & (R - 1)) == 0, "R must be a power of two <= 32");
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -79,6 +152,7 @@ DPS_HD void fft_reg(T (&re)[R], T (&im)[R])
         im[i + k] += bi;
       }
     }
+  }
   }
 }
 

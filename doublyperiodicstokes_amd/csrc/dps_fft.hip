@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const T* __restrict__ in, typ
   if (!ok) return;
   V* x = out + (size_t)row * (M + 1);
   for (int k = t; k <= M; k += TR) {
-    const V p = b[k & (M - 1)], q = b[(M - k) & (M - 1)];
+    const V p = b[k == M ? 0 : k], q = b[k == 0 ? 0 : M - k];   // Z[k mod M], Z[(M - k) mod M]
     const T er = (T)0.5 * (p.x + q.x), ei = (T)0.5 * (p.y - q.y);
     const T orr = (T)0.5 * (p.y + q.y), oi = (T)0.5 * (q.x - p.x);
     const V w = tw[k];
@@ -331,7 +331,13 @@ void fft_x_launch(const void* in, void* out, int nrows, const void* tw, int dir,
 
 bool fft_x_supported(int Nx)
 {
-  return Nx == 32 || Nx == 64 || Nx == 128 || Nx == 256 || Nx == 512 || Nx == 1024 || Nx == 2048;
+  switch (Nx) {
+    case 32: case 64: case 96: case 128: case 192: case 256: case 384: case 512: case 768: case 1024:
+    case 1536: case 2048:
+      return true;
+    default:
+      return false;
+  }
 }
 
 template <class T>
@@ -340,10 +346,15 @@ void launch_fft_x(const void* in, void* out, int nrows, int Nx, const void* tw, 
   switch (Nx) {
     case 32: fft_x_launch<T, 4, 4>(in, out, nrows, tw, dir, s); break;
     case 64: fft_x_launch<T, 8, 4>(in, out, nrows, tw, dir, s); break;
+    case 96: fft_x_launch<T, 8, 6>(in, out, nrows, tw, dir, s); break;
     case 128: fft_x_launch<T, 8, 8>(in, out, nrows, tw, dir, s); break;
+    case 192: fft_x_launch<T, 12, 8>(in, out, nrows, tw, dir, s); break;
     case 256: fft_x_launch<T, 16, 8>(in, out, nrows, tw, dir, s); break;
+    case 384: fft_x_launch<T, 16, 12>(in, out, nrows, tw, dir, s); break;
     case 512: fft_x_launch<T, 16, 16>(in, out, nrows, tw, dir, s); break;
+    case 768: fft_x_launch<T, 24, 16>(in, out, nrows, tw, dir, s); break;
     case 1024: fft_x_launch<T, 32, 16>(in, out, nrows, tw, dir, s); break;
+    case 1536: fft_x_launch<T, 32, 24>(in, out, nrows, tw, dir, s); break;
     case 2048: fft_x_launch<T, 32, 32>(in, out, nrows, tw, dir, s); break;
     default: break;   // callers check fft_x_supported
   }
@@ -353,7 +364,13 @@ template void launch_fft_x<double>(const void*, void*, int, int, const void*, in
 
 bool fft_y_supported(int Ny)
 {
-  return Ny == 16 || Ny == 32 || Ny == 64 || Ny == 128 || Ny == 256 || Ny == 512 || Ny == 1024;
+  switch (Ny) {
+    case 16: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512: case 768:
+    case 1024:
+      return true;
+    default:
+      return false;
+  }
 }
 
 template <class T>
@@ -362,7 +379,12 @@ void launch_fft_y(void* data, int nslices, int Nkx, int Ny, const void* tw, int 
   switch (Ny) {
     case 16: fft_y_launch<T, 4, 4>(data, nslices, Nkx, tw, dir, s); break;
     case 32: fft_y_launch<T, 8, 4>(data, nslices, Nkx, tw, dir, s); break;
+    case 48: fft_y_launch<T, 8, 6>(data, nslices, Nkx, tw, dir, s); break;
     case 64: fft_y_launch<T, 8, 8>(data, nslices, Nkx, tw, dir, s); break;
+    case 96: fft_y_launch<T, 12, 8>(data, nslices, Nkx, tw, dir, s); break;
+    case 192: fft_y_launch<T, 16, 12>(data, nslices, Nkx, tw, dir, s); break;
+    case 384: fft_y_launch<T, 24, 16>(data, nslices, Nkx, tw, dir, s); break;
+    case 768: fft_y_launch<T, 32, 24>(data, nslices, Nkx, tw, dir, s); break;
     case 128: fft_y_launch<T, 16, 8>(data, nslices, Nkx, tw, dir, s); break;
     case 256: fft_y_launch<T, 16, 16>(data, nslices, Nkx, tw, dir, s); break;
     case 512: fft_y_launch<T, 32, 16>(data, nslices, Nkx, tw, dir, s); break;

@@ -1,7 +1,7 @@
-"""The planar transforms with the y pass in k_fft_y (every power-of-two Ny the
-kernel handles, odd and even Nx so that the flattened (slice, kx) column tiles
-straddle slices at different offsets) and, for power-of-two Nx, the x pass in
-k_fft_x (every length it handles), against the FP64 oracle, through Mdot
+"""The planar transforms with the y pass in k_fft_y (every Ny the kernel
+handles, powers of two and 3 * 2^k, odd and even Nx so that the flattened
+(slice, kx) column tiles straddle slices at different offsets) and the x pass
+in k_fft_x (every length it handles), against the FP64 oracle, through Mdot
 with torques.  Bounds as everywhere: rel-L2 <= 1e-11 FP64, <= 1e-5 mixed."""
 import numpy as np
 import pytest
@@ -23,6 +23,15 @@ CASES = [
     ("fx64", "slit", 32.0, 16.0, 6.3, 64, 32, 13, 30, True),
     ("fx512", "bottom", 256.0, 8.0, 6.0, 512, 16, 11, 30, True),
     ("fx2048", "nowall", 1024.0, 8.0, 6.0, 2048, 16, 9, 30, True),
+    # sizes 3 * 2^k: radix-3 register transforms in either pass
+    ("fy48", "slit", 6.5, 24.0, 6.3, 13, 48, 13, 20, True),
+    ("fxy96", "bottom", 48.0, 48.0, 6.0, 96, 96, 11, 30, True),
+    ("fxy192", "slit", 96.0, 96.0, 6.3, 192, 192, 13, 40, True),
+    ("fx384", "nowall", 192.0, 8.0, 6.0, 384, 16, 9, 30, True),
+    ("fy384", "bottom", 8.0, 192.0, 6.0, 16, 384, 9, 30, True),
+    ("fx768", "slit", 384.0, 8.0, 6.3, 768, 16, 13, 30, True),
+    ("fy768", "bottom", 8.0, 384.0, 6.0, 16, 768, 9, 30, True),
+    ("fx1536", "nowall", 768.0, 8.0, 6.0, 1536, 16, 9, 30, True),
 ]
 
 
