@@ -224,11 +224,13 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const T* __restrict__ in, typ
   }
 }
 
-// Inverse: Z is formed from each thread's pairs (X[k], X[M - k]), read from
-// HBM (the second read of a row hits the cache), then the two passes run with
-// the final values written from registers.  (Tried and not kept: the row
-// staged in LDS with Z formed in place, and the radices swapped so that the
-// second pass writes longer runs; both slower.)
+// Inverse: the first pass forms its inputs Z[n1 + R1 n2] from the pairs
+// (X[k], X[M - k]) read from HBM in its own order (runs of R1 values, the
+// mirrored reads hit the cache), then the two passes run as in the forward
+// kernel, the final values written from registers.  (Tried and not kept: Z
+// staged in LDS before the first pass, with the row read once or per pair,
+// and the radices swapped so that the second pass writes longer runs; all
+// slower.)
 template <class T, int R1, int R2>
 __global__ void __launch_bounds__(256) k_fft_x_inv(const typename C2<T>::t* __restrict__ in, T* __restrict__ out,
                                                    int nrows, const typename C2<T>::t* __restrict__ tw)
@@ -236,14 +238,19 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const typename C2<T>::t* __re
   typedef typename C2<T>::t V;
   typedef FftX<T, R1, R2> S;
   constexpr int M = S::M, TR = S::TR, RW = S::RW;
-  __shared__ V buf[RW * M];   // [row][k] = Z[k], then [row][n1][k2] after the first pass
+  __shared__ V buf[RW * M];   // [row][n1][k2] after the first pass
   const int lr = threadIdx.x / TR, t = threadIdx.x % TR;
   const int row = xcd_group(blockIdx.x, gridDim.x) * RW + lr;
   const bool ok = row < nrows;
   V* b = buf + lr * M;
-  {
+  if (t < R1) {
     const V* X = in + (size_t)(ok ? row : 0) * (M + 1);
-    for (int k = t; k < M; k += TR) {
+    const int n1 = t;
+    T re[R2], im[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      // Z[k] = (p + conj q) + i W^{-k} (p - conj q), p = X[k], q = X[M - k]
+      const int k = n1 + R1 * n2;
       V p, q;
       p.x = p.y = q.x = q.y = (T)0;
       if (ok) {
@@ -255,26 +262,10 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const typename C2<T>::t* __re
       const T dr = p.x - q.x, di = p.y + q.y;
       const V w = tw[k];   // W^{-k} = conj(w)
       const T orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
-      V z;
-      z.x = er - oi;
-      z.y = ei + orr;
-      b[k] = z;
-    }
-  }
-  __syncthreads();
-  T re[R2], im[R2];
-  if (t < R1) {
-#pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) {
-      const V v = b[t + R1 * n2];
-      re[n2] = v.x;
-      im[n2] = v.y;
+      re[n2] = er - oi;
+      im[n2] = ei + orr;
     }
     fft_reg<R2, 1>(re, im);
-  }
-  __syncthreads();
-  if (t < R1) {
-    const int n1 = t;
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) {
       V o;
