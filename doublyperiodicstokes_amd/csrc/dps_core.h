@@ -692,6 +692,7 @@ DPS_HD void row_eliminate(const Row4& x, Row4& y, int col, double inv)
 DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx& x0, cplx& x1, cplx& x2,
                          cplx& x3)
 {
+  double inv0, inv1, inv2;
   // column 0
   {
     int piv = 0;
@@ -702,10 +703,10 @@ DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx& x0, cplx&
     row_swap_if(piv == 1, R0, R1);
     row_swap_if(piv == 2, R0, R2);
     row_swap_if(piv == 3, R0, R3);
-    const double inv = 1.0 / R0.a0;
-    row_eliminate(R0, R1, 0, inv);
-    row_eliminate(R0, R2, 0, inv);
-    row_eliminate(R0, R3, 0, inv);
+    inv0 = 1.0 / R0.a0;
+    row_eliminate(R0, R1, 0, inv0);
+    row_eliminate(R0, R2, 0, inv0);
+    row_eliminate(R0, R3, 0, inv0);
   }
   // column 1
   {
@@ -715,21 +716,22 @@ DPS_HD void solve4_pivot(Row4& R0, Row4& R1, Row4& R2, Row4& R3, cplx& x0, cplx&
     if (fabs(R3.a1) > best) { best = fabs(R3.a1); piv = 3; }
     row_swap_if(piv == 2, R1, R2);
     row_swap_if(piv == 3, R1, R3);
-    const double inv = 1.0 / R1.a1;
-    row_eliminate(R1, R2, 1, inv);
-    row_eliminate(R1, R3, 1, inv);
+    inv1 = 1.0 / R1.a1;
+    row_eliminate(R1, R2, 1, inv1);
+    row_eliminate(R1, R3, 1, inv1);
   }
   // column 2
   {
     const bool sw = fabs(R3.a2) > fabs(R2.a2);
     row_swap_if(sw, R2, R3);
-    const double inv = 1.0 / R2.a2;
-    row_eliminate(R2, R3, 2, inv);
+    inv2 = 1.0 / R2.a2;
+    row_eliminate(R2, R3, 2, inv2);
   }
-  x3 = R3.b / R3.a3;
-  x2 = (R2.b - R2.a3 * x3) / R2.a2;
-  x1 = ((R1.b - R1.a2 * x2) - R1.a3 * x3) / R1.a1;
-  x0 = (((R0.b - R0.a1 * x1) - R0.a2 * x2) - R0.a3 * x3) / R0.a0;
+  // back substitution with the pivots' reciprocals (one division each)
+  x3 = R3.b * (1.0 / R3.a3);
+  x2 = (R2.b - R2.a3 * x3) * inv2;
+  x1 = ((R1.b - R1.a2 * x2) - R1.a3 * x3) * inv1;
+  x0 = (((R0.b - R0.a1 * x1) - R0.a2 * x2) - R0.a3 * x3) * inv0;
 }
 
 // Wall-correction coefficients of the bottom wall (DPTools.cpp:617-633),
@@ -738,10 +740,11 @@ DPS_HD void bottom_coeffs(double kx, double ky, const cplx* wall_bot, cplx& c00,
                           cplx& c11, cplx& c20, cplx& c21)
 {
   const double K = sqrt(ksq_of(kx, ky));
+  const double iK = 1.0 / K;
   const cplx fx = -wall_bot[0], fy = -wall_bot[1], fz = -wall_bot[2];
   const cplx alpha = kx * fx + ky * fy + times_i(K * fz);
-  c00 = fx; c01 = ((-kx) * alpha) / K;
-  c10 = fy; c11 = ((-ky) * alpha) / K;
+  c00 = fx; c01 = ((-kx) * alpha) * iK;
+  c10 = fy; c11 = ((-ky) * alpha) * iK;
   c20 = fz; c21 = -times_i(alpha);
 }
 
@@ -770,13 +773,14 @@ DPS_HD void slit_unknowns(const BvpConsts& B, double kx, double ky, const cplx* 
   Row4 R3 = {Hs * enkh * fac, Hs * fac, enkh, 1, rhs7};
   solve4_pivot(R0, R1, R2, R3, X0, X1, X6, X7);
   // rows 2,5: x2 + enkh x3 = b2 ; enkh x2 + x3 = b5 - (A50 X0 + A51 X1)
-  const double d2 = 1 - enkh * enkh;
-  const cplx r5 = rhs5 - (times_i(-kx * Hs * enkh * fac / k * X0) + times_i(kx * Hs * fac / k * X1));
-  X2 = (rhs2 - enkh * r5) / d2;
-  X3 = (r5 - enkh * rhs2) / d2;
-  const cplx r6 = rhs6 - (times_i(-ky * Hs * enkh * fac / k * X0) + times_i(ky * Hs * fac / k * X1));
-  X4 = (rhs3 - enkh * r6) / d2;
-  X5 = (r6 - enkh * rhs3) / d2;
+  const double id2 = 1.0 / (1 - enkh * enkh);
+  const double fk = fac * (1.0 / k);
+  const cplx r5 = rhs5 - (times_i(-kx * Hs * enkh * fk * X0) + times_i(kx * Hs * fk * X1));
+  X2 = (rhs2 - enkh * r5) * id2;
+  X3 = (r5 - enkh * rhs2) * id2;
+  const cplx r6 = rhs6 - (times_i(-ky * Hs * enkh * fk * X0) + times_i(ky * Hs * fk * X1));
+  X4 = (rhs3 - enkh * r6) * id2;
+  X5 = (r6 - enkh * rhs3) * id2;
 }
 
 // Slit coefficients c[comp][basis] from the unknowns.
@@ -788,10 +792,11 @@ DPS_HD void slit_coeffs(const BvpConsts& B, double kx, double ky, const cplx* wa
   slit_unknowns(B, kx, ky, wall_bot, wall_top, X0, X1, X2, X3, X4, X5, X6, X7);
   const double k = sqrt(ksq_of(kx, ky));
   const double fac = 1.0 / 2.0 / B.eta;
+  const double fk = fac * (1.0 / k);
   c00 = X2; c01 = X3;
-  c02 = times_i(-(kx * fac / k) * X0); c03 = times_i((kx * fac / k) * X1);
+  c02 = times_i(-(kx * fk) * X0); c03 = times_i((kx * fk) * X1);
   c10 = X4; c11 = X5;
-  c12 = times_i(-(ky * fac / k) * X0); c13 = times_i((ky * fac / k) * X1);
+  c12 = times_i(-(ky * fk) * X0); c13 = times_i((ky * fk) * X1);
   c20 = X6; c21 = X7;
   c22 = fac * X0; c23 = fac * X1;
 }
