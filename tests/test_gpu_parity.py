@@ -31,6 +31,8 @@ CASES = [
     ("slit_odd", "slit", 13.5, 15.5, 8.0, 27, 31, 25, 40, True),
     ("nowall_noT", "nowall", 12.0, 16.0, 9.0, 24, 32, 20, 30, False),
     ("slit_nz96", "slit", 16.0, 16.0, 10.0, 32, 32, 96, 48, True),      # 16 lanes x 6 rows
+    ("bottom_nz48", "bottom", 16.0, 16.0, 8.0, 32, 32, 48, 40, True),   # 8 lanes x 6 rows
+    ("slit_nz14", "slit", 16.0, 16.0, 6.0, 32, 32, 14, 40, True),       # 8 lanes x 2 rows
     ("bottom_nz150", "bottom", 12.0, 12.0, 9.0, 24, 24, 150, 30, True),  # 32 lanes x 8 rows
     ("bottom_nz120", "bottom", 12.0, 12.0, 9.0, 24, 24, 120, 30, True),  # 16 lanes x 8 rows
     # 64 lanes x 8 rows; a tall box (the reference's Nz rule gives ~300 for
