@@ -68,7 +68,9 @@ __device__ __forceinline__ double dpp64(double v, double fill)
 // Lane exchange of one mode's group.  A 16-lane group is exactly one DPP row:
 // shifts are row_shr/row_shl and the butterfly partners are quad_perm
 // [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror and row_mirror, all VALU
-// operations.  Wider groups use lane permutes.
+// operations.  An 8-lane group is half a row: the same shifts, with the lanes
+// whose partner lies in the other half taking the fill by a select, and the
+// first three butterfly partners.  Wider groups use lane permutes.
 template <int LPM>
 struct DevLanes {
   int l;
@@ -76,7 +78,10 @@ struct DevLanes {
   __device__ double up(double v, double fill) const
   {
     if constexpr (LPM == 16) return dpp64<0x110 + D>(v, fill);   // row_shr:D
-    else {
+    else if constexpr (LPM == 8) {
+      const double r = dpp64<0x110 + D>(v, fill);
+      return (l >= D) ? r : fill;
+    } else {
       const double r = __shfl_up(v, (unsigned)D, LPM);
       return (l >= D) ? r : fill;
     }
@@ -85,7 +90,10 @@ struct DevLanes {
   __device__ double down(double v, double fill) const
   {
     if constexpr (LPM == 16) return dpp64<0x100 + D>(v, fill);   // row_shl:D
-    else {
+    else if constexpr (LPM == 8) {
+      const double r = dpp64<0x100 + D>(v, fill);
+      return (l + D < LPM) ? r : fill;
+    } else {
       const double r = __shfl_down(v, (unsigned)D, LPM);
       return (l + D < LPM) ? r : fill;
     }
@@ -93,7 +101,8 @@ struct DevLanes {
   template <int D>
   __device__ double xr(double v) const
   {
-    if constexpr (LPM == 16) {
+    if constexpr (LPM == 16 || LPM == 8) {
+      static_assert(D < LPM, "butterfly partner outside the group");
       if constexpr (D == 1) return dpp64<0xB1>(v, v);         // quad_perm [1,0,3,2]
       else if constexpr (D == 2) return dpp64<0x4E>(v, v);    // quad_perm [2,3,0,1]
       else if constexpr (D == 4) return dpp64<0x141>(v, v);   // row_half_mirror
