@@ -161,6 +161,7 @@ struct dps_handle {
   bool fused = false;
   int sp_lpm = 0, sp_e = 0, sp_G = 0;
   void *d_TfP = nullptr, *d_TiP = nullptr;
+  float *d_TfM = nullptr, *d_TiM = nullptr;
   double *d_fib = nullptr, *d_fab = nullptr, *d_fG = nullptr;
   int* d_fidx = nullptr;   // [Nm] factor-table entry of each mode
   int n_fac = 0;           // distinct |k|^2 (factor-table entries)
@@ -620,6 +621,28 @@ int allocate_fused(dps_handle* h)
             inv_nxy * ((k & 1) ? h->ct.TiO[(size_t)j * h->ct.NO + k / 2] : h->ct.TiE[(size_t)j * h->ct.NE + k / 2]);
       }
 
+  if (spectral_mfma(LPM, h->prec == DPS_PREC_MIXED)) {
+    // matrix-core operands in v_mfma 16x16x4 lane order: lane L of step s
+    // of row tile t holds row 16 t + L % 16 and term 4 s + L / 16 of the
+    // parity's table, zero outside it and for the steps past its terms
+    const int MT = (LPM * E / 2 + 15) / 16, KSM = LPM * E / 8, NE = h->ct.NE, NO = h->ct.NO;
+    std::vector<float> fm((size_t)2 * MT * KSM * 64, 0.f), im((size_t)2 * MT * KSM * 64, 0.f);
+    for (int par = 0; par < 2; ++par)
+      for (int t = 0; t < MT; ++t)
+        for (int ks = 0; ks < KSM; ++ks)
+          for (int L = 0; L < 64; ++L) {
+            const size_t o = (((size_t)par * MT + t) * KSM + ks) * 64 + L;
+            const int r = 16 * t + L % 16, q = 4 * ks + L / 16;
+            // forward: coefficient 2 r + par of node pair q
+            if (2 * r + par < Nz && q < NH)
+              fm[o] = (float)(par ? h->ct.TfO[(size_t)r * NH + q] : h->ct.TfE[(size_t)r * NH + q]);
+            // inverse: node pair r of coefficient 2 q + par
+            if (r < NH && q < (par ? NO : NE))
+              im[o] = (float)(inv_nxy * (par ? h->ct.TiO[(size_t)r * NO + q] : h->ct.TiE[(size_t)r * NE + q]));
+          }
+    if ((rc = h->upload(&h->d_TfM, fm))) return rc;
+    if ((rc = h->upload(&h->d_TiM, im))) return rc;
+  }
   if (h->prec == DPS_PREC_MIXED) {
     if ((rc = h->upload((float**)&h->d_TfP, cast_vec<float>(tf)))) return rc;
     if ((rc = h->upload((float**)&h->d_TiP, cast_vec<float>(ti)))) return rc;
@@ -963,6 +986,7 @@ SpecArgs spec_args(dps_handle* h)
   A.NH = h->ct.NH;
   A.lpm = h->sp_lpm; A.e = h->sp_e; A.G = h->sp_G;
   A.TfP = h->d_TfP; A.TiP = h->d_TiP;
+  A.TfM = h->d_TfM; A.TiM = h->d_TiM;
   A.fac_idx = h->d_fidx; A.fac_ib = h->d_fib; A.fac_ab = h->d_fab; A.fac_G = h->d_fG;
   A.kxs = h->d_kxs; A.kys = h->d_kys;
   A.zpts = h->d_zpts;

@@ -236,6 +236,12 @@ struct SpecArgs {
   int lpm, e, G;          // lanes per mode, rows per lane, modes per block
   const void* TfP;        // [NH][LPM][E] forward rows in lane order (grid precision)
   const void* TiP;        // [ceil(Nz/2)][LPM][E] inverse rows in lane order (scaled by 1/(Nx Ny))
+  // matrix-core operands of the FP32 transforms (16-lane shapes), in
+  // v_mfma 16x16x4 lane order, zero-padded to LPM E / 8 steps: [2][MT][KSM][64]
+  // forward (parity, coefficient tile, 4 node pairs) and inverse (parity,
+  // node-pair tile, 4 coefficients of that parity; scaled by 1/(Nx Ny))
+  const float* TfM;
+  const float* TiM;
   const int* fac_idx;     // [Nm] entry of each mode in the factor tables (its |k|^2)
   const double* fac_ib;   // [nk][LPM][E] reciprocal pivots of each lane's rows
   const double* fac_ab;   // [nk][LPM][E] A^{-1}B of each lane's rows
@@ -246,11 +252,18 @@ struct SpecArgs {
 };
 // lane-group shape for Nz rows: returns false when Nz is too large
 bool spectral_shape(int Nz, int* lpm, int* e);
+// FP32 z transforms on the matrix cores: 16-lane groups of a mixed-precision
+// solve (16 modes per workgroup).  8-lane groups keep the VALU transforms:
+// their 24-row parities pad to 32-row tiles and the preloaded tables push
+// the solve into scratch (Nz = 48: 836 against 771 us)
+constexpr bool spectral_mfma(int lpm, bool mixed) { return mixed && lpm == 16; }
 // LDS of one spectral workgroup: 9 row tables of LPM E doubles, 12 wall
-// corrections per mode, 3 components x LPM E rows x (G + 1) columns
+// corrections per mode (and |k| per mode on the matrix-core path),
+// 3 components x LPM E rows x (G + 1) columns
 constexpr size_t spectral_lds_bytes(int lpm, int e, int G, bool mixed)
 {
-  return (size_t)9 * lpm * e * 8 + (size_t)G * 12 * 16 + (size_t)3 * lpm * e * (G + 1) * (mixed ? 8 : 16);
+  return (size_t)9 * lpm * e * 8 + (size_t)G * (12 * 16 + (spectral_mfma(lpm, mixed) ? 8 : 0)) +
+         (size_t)3 * lpm * e * (G + 1) * (mixed ? 8 : 16);
 }
 // modes per workgroup: the most (up to 256 lanes) within 64 KiB of LDS, else 1
 constexpr int spectral_modes_per_block(int lpm, int e, bool mixed)

@@ -9,13 +9,17 @@
 //   phase 0  the row tables of the solve are copied to LDS (zero past Nz), and
 //            the G columns of the r2c spectrum, all z levels and components,
 //            are read once from HBM (rows of G modes) into LDS
-//   phase 1  forward transform: lane l forms the coefficients of its rows
-//            from even/odd folded node pairs (grid precision), written back
-//            in place
+//   phase 1  forward transform from even/odd folded node pairs (grid
+//            precision), written back in place: FP32 16-lane shapes on the
+//            matrix cores (ZMma), else lane l forms the coefficients of its
+//            own rows
 //   phase 2  FP64 BVP on the lane group, in place
-//   phase 3  inverse transform of node pairs (l + LPM r, Nz-1-l-LPM r) plus
-//            the analytic wall profiles, written back in place
-//   phase 4  the G columns are written back as rows of G modes
+//   phase 3  inverse transform of node pairs plus the analytic wall
+//            profiles: on the matrix cores the even and odd sums go to LDS
+//            and thread (jr, gl) writes node pairs jr + LPM r of mode gl
+//            straight to HBM; else lane l takes the pairs (l + LPM r,
+//            Nz-1-l-LPM r), written back in place, and
+//   phase 4  writes the G columns back as rows of G modes
 //
 // The spectrum crosses HBM once in each direction; everything in between
 // stays in LDS and registers.
@@ -27,6 +31,7 @@
 // (dps_seg.h, seg_bvp_mode's avg).
 #include <hip/hip_runtime.h>
 #include "dps_internal.h"
+#include "dps_mma.h"
 #include "dps_seg.h"
 
 // velocity components sharing each scan of the lane-group solve
@@ -139,6 +144,118 @@ struct LdsStore {
   }
 };
 
+// FP32 z transforms on the matrix cores.  The G columns of a component are
+// blocks of 16 modes; a column block is (16 modes, component c, real or
+// imaginary part), NCB of them per workgroup.  Each transform is two
+// products, one per parity, each a set of 16 x 16 output tiles with one
+// v_mfma 16x16x4 per tile and 4 terms of the sum: wavefront w takes parity
+// w & 1 and half of the column blocks, over every row tile.  The data operand
+// is read from the LDS columns, the table operand from global memory in lane
+// order, padded with zeros to KSM = NZP / 8 steps so that the step loop has a
+// compile-time length and every table load is issued ahead of the products.
+template <int NZP, int G>
+struct ZMma {
+  static constexpr int GS = G + 1;
+  static constexpr int NCB = (G / 16) * 6;         // column blocks
+  static constexpr int CBW = NCB / 2;              // column blocks per wavefront
+  static constexpr int MT = (NZP / 2 + 15) / 16;   // 16-row tiles of one parity
+  static constexpr int KSM = NZP / 8;              // steps of 4 terms
+  typedef Mma<float>::acc acc;
+
+  // float offset in S of this lane's column of block cb (row 0)
+  static __device__ __forceinline__ int col(int cb, int lane)
+  {
+    const int gb = cb / 6, c = (cb % 6) >> 1, part = cb & 1;
+    return 2 * (c * NZP * GS + 16 * gb + (lane & 15)) + part;
+  }
+
+  // D = Tab_P B over this wavefront's tiles; row(ks) is the S row holding
+  // term 4 ks + lane / 16 of the sum (kept in range past the last term, where
+  // the table is zero)
+  template <class Row>
+  static __device__ __forceinline__ void product(const float* __restrict__ tab, const float* S, int P, int cb0,
+                                                 Row row, acc (&d)[MT][CBW])
+  {
+    const int lane = threadIdx.x & 63;
+    const float* at = tab + (size_t)P * MT * KSM * 64 + lane;
+    float a[MT][KSM];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int ks = 0; ks < KSM; ++ks) a[t][ks] = at[(t * KSM + ks) * 64];
+    int co[CBW];
+#pragma unroll
+    for (int i = 0; i < CBW; ++i) co[i] = col(cb0 + i, lane);
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int i = 0; i < CBW; ++i) d[t][i] = acc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KSM; ++ks) {
+      const int r = row(ks);
+      float b[CBW];
+#pragma unroll
+      for (int i = 0; i < CBW; ++i) b[i] = S[co[i] + 2 * r * GS];
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int i = 0; i < CBW; ++i) d[t][i] = Mma<float>::step(a[t][ks], b[i], d[t][i]);
+    }
+  }
+
+  // Coefficient k = 2 i + P of every column is sum_j Tf_P[i][j] B_P[j], B_0
+  // the folded sums (row j) and B_1 the folded differences (row Nz-1-j; for
+  // the middle node of an odd Nz the zero padding row NZP-1).  The results
+  // replace the folded values in place.
+  static __device__ __forceinline__ void forward(const float* __restrict__ Tf, float* S, int Nz, int NH)
+  {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int P = wv & 1, cb0 = (wv >> 1) * CBW;
+    acc d[MT][CBW];
+    product(Tf, S, P, cb0,
+            [&](int ks) {
+              const int j = min(4 * ks + (lane >> 4), NH - 1);
+              return P ? ((j == Nz - 1 - j) ? NZP - 1 : Nz - 1 - j) : j;
+            },
+            d);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int i = 0; i < CBW; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int k = 2 * (16 * t + Mma<float>::row(lane, v)) + P;
+          if (k < NZP) S[col(cb0 + i, lane) + 2 * k * GS] = d[t][i][v];
+        }
+  }
+
+  // Node pair (j, Nz-1-j), j < NH: SE = sum_p TiE[j][p] C_{2p} and
+  // SO = sum_p TiO[j][p] C_{2p+1} (tables scaled by 1/(Nx Ny)).  Once every
+  // wavefront has read the coefficients, SE goes to row j and SO to row
+  // NZP/2 + j of each column.
+  static __device__ __forceinline__ void inverse(const float* __restrict__ Ti, float* S, int Nz, int NH)
+  {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int P = wv & 1, cb0 = (wv >> 1) * CBW;
+    const int np = P ? Nz / 2 : (Nz + 1) / 2;   // coefficients of parity P
+    acc d[MT][CBW];
+    product(Ti, S, P, cb0, [&](int ks) { return 2 * min(4 * ks + (lane >> 4), np - 1) + P; }, d);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int i = 0; i < CBW; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = 16 * t + Mma<float>::row(lane, v);
+          if (j < NH) S[col(cb0 + i, lane) + 2 * (P * (NZP / 2) + j) * GS] = d[t][i][v];
+        }
+  }
+};
+
 // The solve: each workgroup takes G consecutive modes (G and the waves per
 // SIMD that LDS allows are fixed by the shape, so LDS offsets are
 // compile-time and the register budget is capped to reach that occupancy).
@@ -158,8 +275,11 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   // fall in different LDS banks
   constexpr int GS = G + 1;
   double* tab = reinterpret_cast<double*>(smem_raw);   // [TB_N][NZP]
+  constexpr bool kMfma = spectral_mfma(LPM, sizeof(T) == 4);
+  static_assert(!kMfma || (NT == 256 && G % 16 == 0), "matrix-core transforms take 4 wavefronts, 16-mode blocks");
   double2* Cr = reinterpret_cast<double2*>(tab + TB_N * NZP);   // [G][3][4] wall corrections
-  V* S = reinterpret_cast<V*>(Cr + 12 * G);            // values, coefficients, values
+  double* Kl = reinterpret_cast<double*>(Cr + 12 * G);          // [G] |k| (matrix-core path)
+  V* S = reinterpret_cast<V*>(Kl + (kMfma ? G : 0));            // values, coefficients, values
   const int g = threadIdx.x / LPM;
   const int l = threadIdx.x % LPM;
   const int r0 = l * E;
@@ -221,8 +341,11 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
   }
   __syncthreads();
 
-  // ---- phase 1: forward transform of this lane's rows ----
-  {
+  // ---- phase 1: forward transform ----
+  // (matrix cores; else each lane forms the coefficients of its own rows)
+  if constexpr (kMfma) {
+    ZMma<NZP, G>::forward(A.TfM, reinterpret_cast<float*>(S), Nz, NH);
+  } else {
     typedef T T2 __attribute__((ext_vector_type(2)));
     T ar[3][E], ai[3][E];
 #pragma unroll
@@ -333,7 +456,79 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
 #pragma unroll
         for (int b = 0; b < 4; ++b) Cr[g * 12 + c * 4 + b] = make_double2(res.corr[c][b].re, res.corr[c][b].im);
   }
-  __syncthreads();
+  // basis functions per component (bottom wall: 2, slit: 4; the k = 0 mode
+  // has zero corrections)
+  const int nb = (A.B.geometry == MODE_BOTTOM) ? 2 : ((A.B.geometry == MODE_SLIT) ? 4 : 0);
+  if constexpr (kMfma) {
+    if (l == 0) Kl[g] = K;
+    __syncthreads();
+
+    // ---- phase 3 (matrix cores): inverse sums, then node pairs out ----
+    // SE and SO of every node pair land in LDS (ZMma::inverse); thread
+    // (row group jr, column gl) then forms the pairs jr + LPM r of mode gl
+    // for all three components, adds the wall profiles and writes the
+    // values to HBM as rows of G modes (no separate column pass)
+    ZMma<NZP, G>::inverse(A.TiM, reinterpret_cast<float*>(S), Nz, NH);
+    const int gl = threadIdx.x % G, jr = threadIdx.x / G;
+    const int mo = (m0 + gl < Nm) ? m0 + gl : -1;
+    double zj[R], zm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = min(jr + LPM * r, NH - 1);
+      zj[r] = A.zpts[j];
+      zm[r] = A.zpts[Nz - 1 - j];
+    }
+    __syncthreads();
+    const double Kg = Kl[gl];
+    const double2* cr = Cr + gl * 12;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int j = jr + LPM * r;
+      if (j >= NH || mo < 0) continue;
+      const int jm = Nz - 1 - j;
+      // z_{Nz-1-j} = Lz - z_j, so e^{K(z - Lz)} at one node of the pair is
+      // e^{-K z} at the other; FP32 exponentials for the FP32 grid
+      double ej = 0, em = 0;
+      if (nb) {
+        ej = (double)expf((float)(-Kg * zj[r]));
+        em = (jm == j) ? ej : (double)expf((float)(-Kg * zm[r]));
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const V se = S[(c * NZP + j) * GS + gl], so = S[(c * NZP + NZP / 2 + j) * GS + gl];
+        V vs, vm;
+        vs.x = se.x + so.x;
+        vs.y = se.y + so.y;
+        vm.x = se.x - so.x;
+        vm.y = se.y - so.y;
+        if (nb) {
+          const double2 c0 = cr[c * 4], c1 = cr[c * 4 + 1];
+          double sr = c0.x * ej, si = c0.y * ej, tr = c0.x * em, ti = c0.y * em;
+          if (nb == 2) {
+            sr += c1.x * (zj[r] * ej);
+            si += c1.y * (zj[r] * ej);
+            tr += c1.x * (zm[r] * em);
+            ti += c1.y * (zm[r] * em);
+          } else {
+            const double2 c2 = cr[c * 4 + 2], c3 = cr[c * 4 + 3];
+            sr += c1.x * em + c2.x * (zj[r] * ej) + c3.x * (zj[r] * em);
+            si += c1.y * em + c2.y * (zj[r] * ej) + c3.y * (zj[r] * em);
+            tr += c1.x * ej + c2.x * (zm[r] * em) + c3.x * (zm[r] * ej);
+            ti += c1.y * ej + c2.y * (zm[r] * em) + c3.y * (zm[r] * ej);
+          }
+          vs.x += (T)(sr * A.inv_nxy);
+          vs.y += (T)(si * A.inv_nxy);
+          vm.x += (T)(tr * A.inv_nxy);
+          vm.y += (T)(ti * A.inv_nxy);
+        }
+        out[(size_t)(c * Nz + j) * Nm + mo] = vs;
+        if (jm != j) out[(size_t)(c * Nz + jm) * Nm + mo] = vm;
+      }
+    }
+    return;
+  } else {
+    __syncthreads();
+  }
 
   // ---- phase 3: inverse transform + wall profiles ----
   {
@@ -380,9 +575,6 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
         }
     }
     __syncthreads();
-    // basis functions per component (bottom wall: 2, slit: 4; the k = 0
-    // mode has zero corrections)
-    const int nb = (A.B.geometry == MODE_BOTTOM) ? 2 : ((A.B.geometry == MODE_SLIT) ? 4 : 0);
     const double2* cr = Cr + g * 12;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
