@@ -403,10 +403,14 @@ def main():
                      "launch_us": stage_us[dom]},
     }
     if kernels["bvp"] == "k_spectral" and not reg:
-        # the Chebyshev z transforms inside the fused solve: FP32 VALU sums
-        # over folded node pairs, forward and inverse
+        # the Chebyshev z transforms inside the fused solve, forward and
+        # inverse over folded node pairs: v_mfma 16x16x4 FP32 products for
+        # 16-lane groups (17 <= Nz <= 32 or 49 <= Nz <= 128), FP32 VALU sums
+        # for the other shapes
         zf = z_transform_flop(wl)
-        out["z_transform"] = {"kernel": "k_spectral", "unit": "VALU FP32", "flop_per_launch": zf,
+        nz = wl[9]
+        unit = "MFMA FP32" if (16 < nz <= 32 or 48 < nz <= 128) else "VALU FP32"
+        out["z_transform"] = {"kernel": "k_spectral", "unit": unit, "flop_per_launch": zf,
                               "tflops_over_kernel_time": zf / (stage_us["bvp"] * 1e-6) / 1e12,
                               "peak_tflops": FP32_PEAK_TFLOPS}
     del mixed
