@@ -31,6 +31,7 @@ DPS_NOALLOC_SCRATCH_BYTES = 268435456
 EXPORTS = ("dps_create", "dps_destroy", "dps_set_positions", "dps_update_positions",
            "dps_check_positions", "dps_mdot",
            "dps_mdot_adjoint", "dps_mdot_adjoint_noalloc", "dps_batch_create", "dps_batch_destroy", "dps_batch_mdot", "dps_batch_tune",
+           "dps_batch_set_chunks",
            "dps_stage_spread",
            "dps_stage_solve", "dps_stage_interp", "dps_get_info", "dps_set_profiling",
            "dps_get_timings", "dps_get_precision", "dps_last_error")
@@ -82,6 +83,7 @@ def load():
     L.dps_batch_destroy.argtypes = [vp]
     L.dps_batch_mdot.argtypes = [vp, pp, pp, pp, pp, ctypes.c_int, vp]
     L.dps_batch_tune.argtypes = [vp, pp, pp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
+    L.dps_batch_set_chunks.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, vp]
     L.dps_stage_spread.argtypes = [vp, vp, vp, ctypes.c_int, vp, vp]
     L.dps_stage_solve.argtypes = [vp, vp, vp, vp]
     L.dps_stage_interp.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]

@@ -1861,6 +1861,31 @@ int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* t
   return rc;
 }
 
+int dps_batch_set_chunks(dps_batch* b, const int* chunks, int torques, void* stream)
+{
+  if (!b) return fail(DPS_ERR_STATE, "null batch");
+  if (!chunks) return fail(DPS_ERR_ARG, "null chunks");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  // every instance is checked before any changes, so a refused call leaves
+  // the batch as it was
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    dps_handle* h = b->hs[i];
+    if ((rc = check_handle(h, true))) return rc;
+    const int zc = chunks[i];
+    if (zc > h->Nz || zc < h->zc_min || !spread_fits(h, torques && h->has_dip, zc))
+      return fail(DPS_ERR_ARG, "spreading z chunk " + std::to_string(zc) + " does not fit instance " +
+                                   std::to_string(i));
+  }
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    dps_handle* h = b->hs[i];
+    h->ZC = chunks[i];
+    if ((rc = build_lists(h, s))) return rc;
+  }
+  b->drop_graph();   // launch shapes changed
+  return DPS_OK;
+}
+
 int dps_get_info(const dps_handle* h, int* out)
 {
   int rc = check_handle(h, false);

@@ -31,7 +31,9 @@ All arithmetic happens in the HIP library; if it is not built this module
 raises on use.
 """
 import ctypes
+import json
 import os
+import tempfile
 
 import numpy as np
 
@@ -369,6 +371,46 @@ class DPStokes:
         return dict(zip(keys, list(out)))
 
 
+def tune_key(solver, torques, dtype):
+    """Cache key of one batch instance: device architecture, every solver
+    parameter, particle count, whether torques are spread, input dtype."""
+    torch = _torch()
+    try:
+        props = torch.cuda.get_device_properties(solver.device)
+        arch = getattr(props, "gcnArchName", "") or props.name
+    except Exception:
+        arch = "unknown"
+    par = {k: v for k, v in sorted(vars(solver.par).items())}
+    return json.dumps({"arch": arch, "par": par, "N": int(solver.N), "torques": bool(torques),
+                       "dtype": str(dtype)}, sort_keys=True, default=str)
+
+
+def load_tune_cache(path):
+    """Stored tuning choices {key: chunk}; empty when the file is absent."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        data = json.load(f)
+    if not isinstance(data, dict) or data.get("version") != 1 or not isinstance(data.get("chunks"), dict):
+        raise ValueError("%s is not a tuning cache of this library" % path)
+    return dict(data["chunks"])
+
+
+def save_tune_cache(path, chunks):
+    """Write {key: chunk} atomically (a temporary file renamed over ``path``)."""
+    d = os.path.dirname(os.path.abspath(path))
+    fd, tmp = tempfile.mkstemp(prefix=".tune-", dir=d)
+    try:
+        with os.fdopen(fd, "w") as f:
+            json.dump({"version": 1, "chunks": {k: int(v) for k, v in chunks.items()}}, f, indent=1,
+                      sort_keys=True)
+        os.replace(tmp, path)
+    except BaseException:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise
+
+
 class DPStokesBatch:
     """Many independent solvers run as one (dps_batch_*): every instance's
     Mdot captured into one HIP graph, forked over ``nstreams`` streams, and
@@ -414,11 +456,26 @@ class DPStokesBatch:
     def _ptrs(ts):
         return (ctypes.c_void_p * len(ts))(*[t.data_ptr() if t is not None else None for t in ts])
 
-    def tune(self, forces, torques=None):
-        """Autotune the spreading z chunk of every instance; returns the chunks."""
+    def tune(self, forces, torques=None, cache=None):
+        """Autotune the spreading z chunk of every instance; returns the chunks.
+
+        With ``cache`` (path of a JSON file), instances whose configuration
+        the file holds take their stored chunk without timing, provided every
+        instance of the batch is found there; otherwise all are timed and the
+        file is updated with the choices.  A choice is measured on the
+        positions set when it was tuned."""
         F, T, dt = self._inputs(forces, torques)
+        keys = [tune_key(s, t is not None, dt) for s, t in zip(self.solvers, T)]
+        stored = load_tune_cache(cache) if cache else {}
+        if cache and all(k in stored for k in keys):
+            chunks = (ctypes.c_int * len(F))(*[int(stored[k]) for k in keys])
+            N.check(N.load().dps_batch_set_chunks(self._b, chunks, int(any(t is not None for t in T)), _stream()))
+            return list(chunks)
         out = (ctypes.c_int * len(F))()
         N.check(N.load().dps_batch_tune(self._b, self._ptrs(F), self._ptrs(T), _dtype_code(dt), _stream(), out))
+        if cache:
+            stored.update(zip(keys, list(out)))
+            save_tune_cache(cache, stored)
         return list(out)
 
     def mdot_device(self, forces, torques=None):

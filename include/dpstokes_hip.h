@@ -213,6 +213,14 @@ int dps_batch_mdot(dps_batch* b, const void* const* forces, const void* const* t
                    void* const* angular, int dtype, void* stream);
 int dps_batch_tune(dps_batch* b, const void* const* forces, const void* const* torques, int dtype, void* stream,
                    int* chunk_out);
+/* Apply spreading z chunks chosen earlier (one int per instance, e.g. what
+   dps_batch_tune returned and the caller stored), without timing: each must
+   fit its instance (with the torque pass when `torques` is nonzero and the
+   instance has a dipole kernel), else DPS_ERR_ARG and nothing changes.
+   Rebuilds the entry lists and drops the captured graph.  No counterpart in
+   the reference: it lets a caller reuse tuned choices across runs (the
+   Python layer's on-disk tuning cache). */
+int dps_batch_set_chunks(dps_batch* b, const int* chunks, int torques, void* stream);
 
 /* Query derived sizes: nx, ny, nz, number of particles, Nzeff of the monopole
    and dipole z-extended grids, z support widths.  `out` holds 8 ints. */

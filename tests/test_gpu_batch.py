@@ -4,6 +4,9 @@ replay -- repeated, with eager work and synchronisations between, after a
 position update, after autotuning -- matches the instances' own eager Mdot
 (FP64 rel-L2 <= 1e-12; mixed <= 1e-6), and one instance matches the oracle
 (<= 1e-11)."""
+import ctypes
+import json
+
 import numpy as np
 import pytest
 
@@ -100,6 +103,48 @@ def test_batch_replays_match_eager(precision):
         e = rel_l2(U[4].cpu().numpy(), v)
         say("  instance 4 vs oracle rel-L2 %.3e" % e)
         assert e <= 1e-11
+    b.clear()
+    for s in solvers:
+        s.clear()
+
+
+def test_batch_tune_cache(tmp_path):
+    """A second tune with the same cache file applies the stored chunks
+    (dps_batch_set_chunks, no timing); replays then still match eager.  A
+    chunk that cannot fit is refused and changes nothing."""
+    import torch
+    from doublyperiodicstokes_amd import DPStokesBatch
+    from doublyperiodicstokes_amd import _native as N
+    say("batch tuning cache")
+    dev = torch.device("cuda", 0)
+    rng = np.random.RandomState(9)
+    insts = INSTANCES[:3]
+    solvers = [_solver(i, "mixed") for i in insts]
+    for s, i in zip(solvers, insts):
+        s.setPositions(_positions(i, rng))
+    F = [torch.tensor(rng.normal(size=3 * i[5]), dtype=torch.float32, device=dev) for i in insts]
+    T = [torch.tensor(rng.normal(size=3 * i[5]), dtype=torch.float32, device=dev) if i[6] else None
+         for i in insts]
+    path = str(tmp_path / "tune.json")
+    b = DPStokesBatch(solvers, nstreams=2)
+    first = b.tune(F, T, cache=path)
+    with open(path) as f:
+        stored = json.load(f)["chunks"]
+    assert sorted(stored.values()) == sorted(first)
+    second = b.tune(F, T, cache=path)
+    say("  tuned %s, from cache %s" % (first, second))
+    assert second == first
+    ref = [s.mdot_device(f, t) for s, f, t in zip(solvers, F, T)]
+    ref = [(u.double().cpu().numpy(), None if w is None else w.double().cpu().numpy()) for u, w in ref]
+    U, W = b.mdot_device(F, T)
+    torch.cuda.synchronize()
+    for k, ((u_ref, w_ref), u, w) in enumerate(zip(ref, U, W)):
+        assert rel_l2(u.double().cpu().numpy(), u_ref) <= 1e-6, k
+        if w_ref is not None:
+            assert rel_l2(w.double().cpu().numpy(), w_ref) <= 1e-6, k
+    bad = (ctypes.c_int * 3)(first[0], 4096, first[2])
+    with pytest.raises(ValueError):
+        N.check(N.load().dps_batch_set_chunks(b._b, bad, 1, None))
     b.clear()
     for s in solvers:
         s.clear()
