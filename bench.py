@@ -28,8 +28,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 HERE = os.path.dirname(os.path.abspath(__file__))
 # kernel that runs each stage (the "bvp" stage is the fused spectral kernel)
-STAGE_KERNEL = {"spread": "k_spread", "fft_fwd": "rocfft x + k_fft_y", "z_fwd": "k_spectral",
-                "bvp": "k_spectral", "z_inv": "k_spectral", "fft_inv": "k_fft_y + rocfft x", "interp": "k_interp"}
+STAGE_KERNEL = {"spread": "k_spread", "fft_fwd": "k_fft_x_fwd + k_fft_y", "z_fwd": "k_spectral",
+                "bvp": "k_spectral", "z_inv": "k_spectral", "fft_inv": "k_fft_y + k_fft_x_inv", "interp": "k_interp"}
 # unfused spectral chain (one wavenumber per lane)
 STAGE_KERNEL_LANE = dict(STAGE_KERNEL, z_fwd="k_zfwd", bvp="k_bvp", z_inv="k_zinv")
 
