@@ -525,125 +525,124 @@ __global__ void __launch_bounds__(spectral_modes_per_block(LPM, E, sizeof(T) == 
         if (jm != j) out[(size_t)(c * Nz + jm) * Nm + mo] = vm;
       }
     }
-    return;
   } else {
     __syncthreads();
-  }
 
-  // ---- phase 3: inverse transform + wall profiles ----
-  {
-    typedef T T2 __attribute__((ext_vector_type(2)));
-    // [ceil(Nz/2)][LPM][E]: for the coefficient pair (2p, 2p+1), the R rows
-    // of coefficient 2p then the R rows of 2p+1, for this lane's node pairs
-    const T2* Ti = (const T2*)A.TiP + l * (E / 2);
-    // coordinates of this lane's node pairs, loaded ahead of the sums
-    double zj[R], zm[R];
+    // ---- phase 3: inverse transform + wall profiles ----
+    {
+      typedef T T2 __attribute__((ext_vector_type(2)));
+      // [ceil(Nz/2)][LPM][E]: for the coefficient pair (2p, 2p+1), the R rows
+      // of coefficient 2p then the R rows of 2p+1, for this lane's node pairs
+      const T2* Ti = (const T2*)A.TiP + l * (E / 2);
+      // coordinates of this lane's node pairs, loaded ahead of the sums
+      double zj[R], zm[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int j = (l + LPM * r < NH) ? l + LPM * r : NH - 1;
-      zj[r] = A.zpts[j];
-      zm[r] = A.zpts[Nz - 1 - j];
-    }
-    T er[3][R], ei[3][R], orr[3][R], oi[3][R];
+      for (int r = 0; r < R; ++r) {
+        const int j = (l + LPM * r < NH) ? l + LPM * r : NH - 1;
+        zj[r] = A.zpts[j];
+        zm[r] = A.zpts[Nz - 1 - j];
+      }
+      T er[3][R], ei[3][R], orr[3][R], oi[3][R];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int r = 0; r < R; ++r) er[c][r] = ei[c][r] = orr[c][r] = oi[c][r] = (T)0;
+        for (int r = 0; r < R; ++r) er[c][r] = ei[c][r] = orr[c][r] = oi[c][r] = (T)0;
 #pragma unroll 4
-    for (int p = 0; p < Nz; p += 2) {
-      V u[3], w[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        u[c] = S[(c * NZP + p) * GS + g];
-        w[c] = S[(c * NZP + p + 1) * GS + g];   // a zero padding row when p + 1 = Nz
-      }
-      T t[E];
-#pragma unroll
-      for (int q = 0; q < E; q += 2) {
-        const T2 v = Ti[(size_t)(p / 2) * (LPM * E / 2) + q / 2];
-        t[q] = v.x;
-        t[q + 1] = v.y;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r)
+      for (int p = 0; p < Nz; p += 2) {
+        V u[3], w[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          er[c][r] += t[r] * u[c].x;
-          ei[c][r] += t[r] * u[c].y;
-          orr[c][r] += t[R + r] * w[c].x;
-          oi[c][r] += t[R + r] * w[c].y;
+          u[c] = S[(c * NZP + p) * GS + g];
+          w[c] = S[(c * NZP + p + 1) * GS + g];   // a zero padding row when p + 1 = Nz
         }
+        T t[E];
+#pragma unroll
+        for (int q = 0; q < E; q += 2) {
+          const T2 v = Ti[(size_t)(p / 2) * (LPM * E / 2) + q / 2];
+          t[q] = v.x;
+          t[q + 1] = v.y;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            er[c][r] += t[r] * u[c].x;
+            ei[c][r] += t[r] * u[c].y;
+            orr[c][r] += t[R + r] * w[c].x;
+            oi[c][r] += t[R + r] * w[c].y;
+          }
+      }
+      __syncthreads();
+      const double2* cr = Cr + g * 12;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = l + LPM * r;
+        if (j >= NH) continue;
+        const int jm = Nz - 1 - j;
+        // z_{Nz-1-j} = Lz - z_j, so e^{K(z - Lz)} at one node of the pair is
+        // e^{-K z} at the other.  The profile lands in grid precision, so an
+        // FP32 grid takes FP32 exponentials (the coefficients stay FP64).
+        double ej = 0, em = 0;
+        if (nb) {
+          if constexpr (sizeof(T) == 4) {
+            ej = (double)expf((float)(-K * zj[r]));
+            em = (jm == j) ? ej : (double)expf((float)(-K * zm[r]));
+          } else {
+            ej = exp(-K * zj[r]);
+            em = (jm == j) ? ej : exp(-K * zm[r]);
+          }
+        }
+        for (int side = 0; side < 2; ++side) {
+          const int jj = side ? jm : j;
+          if (side && jj == j) break;
+          const double z = side ? zm[r] : zj[r];
+          const double e1 = side ? em : ej;
+          const double e2 = side ? ej : em;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            V v;
+            v.x = side ? (er[c][r] - orr[c][r]) : (er[c][r] + orr[c][r]);
+            v.y = side ? (ei[c][r] - oi[c][r]) : (ei[c][r] + oi[c][r]);
+            if (nb) {
+              const double2 c0 = cr[c * 4], c1 = cr[c * 4 + 1];
+              double sr = c0.x * e1, si = c0.y * e1;
+              if (nb == 2) {
+                sr += c1.x * (z * e1);
+                si += c1.y * (z * e1);
+              } else {
+                const double2 c2 = cr[c * 4 + 2], c3 = cr[c * 4 + 3];
+                sr += c1.x * e2 + c2.x * (z * e1) + c3.x * (z * e2);
+                si += c1.y * e2 + c2.y * (z * e1) + c3.y * (z * e2);
+              }
+              v.x += (T)(sr * A.inv_nxy);
+              v.y += (T)(si * A.inv_nxy);
+            }
+            S[(c * NZP + jj) * GS + g] = v;
+          }
+        }
+      }
     }
     __syncthreads();
-    const double2* cr = Cr + g * 12;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int j = l + LPM * r;
-      if (j >= NH) continue;
-      const int jm = Nz - 1 - j;
-      // z_{Nz-1-j} = Lz - z_j, so e^{K(z - Lz)} at one node of the pair is
-      // e^{-K z} at the other.  The profile lands in grid precision, so an
-      // FP32 grid takes FP32 exponentials (the coefficients stay FP64).
-      double ej = 0, em = 0;
-      if (nb) {
-        if constexpr (sizeof(T) == 4) {
-          ej = (double)expf((float)(-K * zj[r]));
-          em = (jm == j) ? ej : (double)expf((float)(-K * zm[r]));
-        } else {
-          ej = exp(-K * zj[r]);
-          em = (jm == j) ? ej : exp(-K * zm[r]);
-        }
-      }
-      for (int side = 0; side < 2; ++side) {
-        const int jj = side ? jm : j;
-        if (side && jj == j) break;
-        const double z = side ? zm[r] : zj[r];
-        const double e1 = side ? em : ej;
-        const double e2 = side ? ej : em;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          V v;
-          v.x = side ? (er[c][r] - orr[c][r]) : (er[c][r] + orr[c][r]);
-          v.y = side ? (ei[c][r] - oi[c][r]) : (ei[c][r] + oi[c][r]);
-          if (nb) {
-            const double2 c0 = cr[c * 4], c1 = cr[c * 4 + 1];
-            double sr = c0.x * e1, si = c0.y * e1;
-            if (nb == 2) {
-              sr += c1.x * (z * e1);
-              si += c1.y * (z * e1);
-            } else {
-              const double2 c2 = cr[c * 4 + 2], c3 = cr[c * 4 + 3];
-              sr += c1.x * e2 + c2.x * (z * e1) + c3.x * (z * e2);
-              si += c1.y * e2 + c2.y * (z * e1) + c3.y * (z * e2);
-            }
-            v.x += (T)(sr * A.inv_nxy);
-            v.y += (T)(si * A.inv_nxy);
-          }
-          S[(c * NZP + jj) * GS + g] = v;
-        }
-      }
-    }
-  }
-  __syncthreads();
 
-  // ---- phase 4: columns out ----
-  // (the LDS rows of a thread are read together, then stored; the thread
-  // index is re-read opaquely so the row offsets are not shared with phase 0
-  // and kept live through the solve)
-  {
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    const int gc4 = tid % G;
-    const int mc4 = (m0 + gc4 < Nm) ? m0 + gc4 : -1;
-    constexpr int NL = 3 * E;
-    V cv[NL];
+    // ---- phase 4: columns out ----
+    // (the LDS rows of a thread are read together, then stored; the thread
+    // index is re-read opaquely so the row offsets are not shared with phase 0
+    // and kept live through the solve)
+    {
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      const int gc4 = tid % G;
+      const int mc4 = (m0 + gc4 < Nm) ? m0 + gc4 : -1;
+      constexpr int NL = 3 * E;
+      V cv[NL];
 #pragma unroll
-    for (int k = 0; k < NL; ++k) cv[k] = S[(tid / G + k * LPM) * GS + gc4];
+      for (int k = 0; k < NL; ++k) cv[k] = S[(tid / G + k * LPM) * GS + gc4];
 #pragma unroll
-    for (int k = 0; k < NL; ++k) {
-      const int row = tid / G + k * LPM;
-      const int c = row / NZP, r = row - c * NZP;
-      if (mc4 >= 0 && r < Nz) out[(size_t)(c * Nz + r) * Nm + mc4] = cv[k];
+      for (int k = 0; k < NL; ++k) {
+        const int row = tid / G + k * LPM;
+        const int c = row / NZP, r = row - c * NZP;
+        if (mc4 >= 0 && r < Nz) out[(size_t)(c * Nz + r) * Nm + mc4] = cv[k];
+      }
     }
   }
 }
