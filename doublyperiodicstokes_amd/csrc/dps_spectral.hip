@@ -242,7 +242,7 @@ struct ZMma {
     const int P = wv & 1, cb0 = (wv >> 1) * CBW;
     const int np = P ? Nz / 2 : (Nz + 1) / 2;   // coefficients of parity P
     acc d[MT][CBW];
-    product(Ti, S, P, cb0, [&](int ks) { return 2 * min(4 * ks + (lane >> 4), np - 1) + P; }, d);
+    product(Ti, S, P, cb0, [&](int ks) { return 2 * min(4 * ks + (lane >> 4), max(np - 1, 0)) + P; }, d);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < MT; ++t)
