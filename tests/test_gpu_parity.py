@@ -41,6 +41,9 @@ CASES = [
     # divides exp(-k Lz) by exp(-k z) (DPTools.cpp:742, 801), 0/0 once
     # k Lz > ~709
     ("bottom_nz300", "bottom", 12.0, 12.0, 96.0, 24, 24, 300, 30, True),
+    # odd Nz on the matrix-core transforms: 16 lanes x 6 and 16 lanes x 4 rows
+    ("slit_nz85", "slit", 16.0, 16.0, 10.0, 32, 32, 85, 40, True),
+    ("bottom_nz57", "bottom", 16.0, 16.0, 10.0, 32, 32, 57, 40, True),
 ]
 
 
@@ -128,7 +131,12 @@ def test_lane_solver_matches_oracle(case, precision):
     s.clear()
 
 
-@pytest.mark.parametrize("case", CASES[:5], ids=[c[0] for c in CASES[:5]])
+# the first five cases, plus the 16-lane shapes whose FP32 transforms run on
+# the matrix cores with even and odd Nz
+MIXED_CASES = CASES[:5] + [c for c in CASES if c[0] in ("slit_nz96", "bottom_nz120", "slit_nz85", "bottom_nz57")]
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
 def test_mdot_mixed_matches_oracle(case):
     say("case %s mixed mdot" % case[0])
     pos, F, T = make_case(case, seed=1)
