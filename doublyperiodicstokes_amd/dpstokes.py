@@ -463,7 +463,8 @@ class DPStokesBatch:
         the file holds take their stored chunk without timing, provided every
         instance of the batch is found there; otherwise all are timed and the
         file is updated with the choices.  A choice is measured on the
-        positions set when it was tuned."""
+        positions set when it was tuned.  A file that is not a tuning cache
+        of this library raises ValueError and is left as it is."""
         F, T, dt = self._inputs(forces, torques)
         keys = [tune_key(s, t is not None, dt) for s, t in zip(self.solvers, T)]
         stored = load_tune_cache(cache) if cache else {}
